@@ -1,0 +1,214 @@
+"""ctypes access to the CPU checker libraries built by ``oracle/Makefile``.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the cpu_baseline leg of bench.py -- as the checker, never as the product.
+
+* ``liborc.so``      -- the CPU restatement of the gray M1 path (FP64)
+* ``liborc_count.so`` -- the same code over a FLOP-counting real type
+* ``_ref/libbnsref.so`` -- the genuine reference compiled from its headers
+  (n <= 10 only: BS_N_MAX = 20 in bns_nurates.hpp:68)
+
+Both ``orc_*`` and ``ref_*`` entry points share one signature per function,
+so every wrapper below takes the library and a prefix.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {
+    "orc": os.path.join(HERE, "liborc.so"),
+    "count": os.path.join(HERE, "liborc_count.so"),
+    "ref": os.path.join(HERE, "_ref", "libbnsref.so"),
+}
+_cache = {}
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i64 = C.c_int64
+
+
+def available(kind: str) -> bool:
+    return os.path.exists(_LIBS[kind])
+
+
+def build() -> None:
+    """Run the oracle Makefile (also builds _ref/ when the reference tree is
+    present)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load(kind: str = "orc"):
+    if kind in _cache:
+        return _cache[kind]
+    path = _LIBS[kind]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
+    lib = C.CDLL(path)
+    p = "ref" if kind == "ref" else "orc"
+    if kind == "count":
+        f = lib.orc_count_m1_opacities
+        f.argtypes = [_i64, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, _ip, _ip, C.c_int, _dp, _i64p]
+        f.restype = C.c_int
+        _cache[kind] = (lib, p)
+        return _cache[kind]
+    sig = {
+        "gauss_legendre": ([C.c_int, C.c_double, C.c_double, _dp, _dp], None),
+        "fdi": ([C.c_int, _i64, _dp, _dp], None),
+        "fermi_distr": ([_i64, _dp, _dp, _dp, _dp], None),
+        "bessk1": ([_i64, _dp, _dp], None),
+        "pair_kernels": ([_i64, _dp, _dp, _dp, _dp, _dp, _dp], None),
+        "brem_brt06": ([_i64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], None),
+        "brem_hr98": ([_i64, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp], None),
+        "inel_kernels": ([_i64, _dp, _dp, _dp, _dp, _dp, _dp], None),
+        "abs_opacity": ([_i64, _dp, _dp, _ip, C.c_int, _dp, _dp], None),
+        "iso_scatt": ([_i64, _dp, _dp, _ip, _dp], None),
+        "distr_from_m1": ([_i64, _dp, _dp], None),
+        "distr_equilibrium": ([_i64, _dp, _dp], None),
+        "m1_densities_eq": ([_i64, _dp, _dp], None),
+        "total_nuf": ([_i64, _dp, _dp, C.c_int, _dp], None),
+        "m1_opacities": ([_i64, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_void_p, _ip,
+                          _ip, C.c_int, _dp, C.c_int], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, f"{p}_{name}")
+        fn.argtypes = args
+        fn.restype = res
+    _cache[kind] = (lib, p)
+    return _cache[kind]
+
+
+def _fn(kind, name):
+    lib, p = load(kind)
+    return getattr(lib, f"{p}_{name}")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def gauss_legendre(n: int, kind: str = "orc", x1: float = 0.0, x2: float = 1.0):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _fn(kind, "gauss_legendre")(n, x1, x2, x, w)
+    return x, w
+
+
+def fdi(k: int, y, kind: str = "orc"):
+    y = _f64(y).ravel()
+    out = np.empty_like(y)
+    _fn(kind, "fdi")(k, y.size, y, out)
+    return out
+
+
+def pair_kernels(om, omp, T, mu_e, kind="orc"):
+    om, omp, T, mu_e = (_f64(a).ravel() for a in np.broadcast_arrays(om, omp, T, mu_e))
+    em = np.empty((om.size, 4))
+    ab = np.empty((om.size, 4))
+    _fn(kind, "pair_kernels")(om.size, om, omp, T, mu_e, em, ab)
+    return em, ab
+
+
+def brem(impl: str, om, omp, nb, T, yn, yp, medium=1, kind="orc"):
+    om, omp, nb, T, yn, yp = (_f64(a).ravel() for a in np.broadcast_arrays(om, omp, nb, T, yn, yp))
+    em = np.empty(om.size)
+    ab = np.empty(om.size)
+    if impl == "brt06":
+        _fn(kind, "brem_brt06")(om.size, om, omp, nb, T, yn, yp, em, ab)
+    else:
+        _fn(kind, "brem_hr98")(om.size, om, omp, nb, T, yn, yp, int(medium), em, ab)
+    return em, ab
+
+
+def inel_kernels(om, omp, T, mu_e, kind="orc"):
+    om, omp, T, mu_e = (_f64(a).ravel() for a in np.broadcast_arrays(om, omp, T, mu_e))
+    em = np.empty((om.size, 4))
+    ab = np.empty((om.size, 4))
+    _fn(kind, "inel_kernels")(om.size, om, omp, T, mu_e, em, ab)
+    return em, ab
+
+
+def abs_opacity(om, eos9, params, stim=1, kind="orc"):
+    om = _f64(om).ravel()
+    em = np.empty((om.size, 2))
+    ab = np.empty((om.size, 2))
+    _fn(kind, "abs_opacity")(om.size, om, _f64(eos9), np.ascontiguousarray(params, np.int32),
+                             int(stim), em, ab)
+    return em, ab
+
+
+def iso_scatt(om, eos9, params, kind="orc"):
+    om = _f64(om).ravel()
+    out = np.empty(om.size)
+    _fn(kind, "iso_scatt")(om.size, om, _f64(eos9), np.ascontiguousarray(params, np.int32), out)
+    return out
+
+
+def distr_from_m1(pts, kind="orc"):
+    pts = _f64(pts)
+    d = np.empty((28, pts.shape[1]))
+    _fn(kind, "distr_from_m1")(pts.shape[1], pts, d)
+    return d
+
+
+def distr_equilibrium(pts, kind="orc"):
+    pts = _f64(pts)
+    d = np.empty((28, pts.shape[1]))
+    _fn(kind, "distr_equilibrium")(pts.shape[1], pts, d)
+    return d
+
+
+def m1_densities_eq(pts, distr, kind="orc"):
+    pts = _f64(pts).copy()
+    _fn(kind, "m1_densities_eq")(pts.shape[1], pts, _f64(distr))
+    return pts
+
+
+def total_nuf(om, distr28, s, kind="orc"):
+    om = _f64(om).ravel()
+    out = np.empty(om.size)
+    _fn(kind, "total_nuf")(om.size, om, _f64(distr28), int(s), out)
+    return out
+
+
+def m1_opacities(pts, nquad, flags, params, stim=1, distr=None, kind="orc", nthreads=0,
+                 nquad_1d=None):
+    """Gray M1 opacities [20, npts] for the point block pts [21, npts]."""
+    pts = _f64(pts)
+    npts = pts.shape[1]
+    n1 = nquad if nquad_1d is None else nquad_1d
+    x1, w1 = gauss_legendre(n1, kind=kind)
+    x2, w2 = gauss_legendre(nquad, kind=kind)
+    out = np.zeros((20, npts))
+    dptr = None
+    if distr is not None:
+        distr = _f64(distr)
+        dptr = distr.ctypes.data_as(C.c_void_p)
+    rc = _fn(kind, "m1_opacities")(npts, n1, x1, w1, nquad, x2, w2, pts, dptr,
+                                   np.ascontiguousarray(flags, np.int32),
+                                   np.ascontiguousarray(params, np.int32), int(stim), out,
+                                   int(nthreads))
+    if rc != 0:
+        raise ValueError(f"{kind}_m1_opacities returned {rc}")
+    return out
+
+
+def count_m1_opacities(pts, nquad, flags, params, stim=1):
+    """Algorithmic operation counts of the gray path over pts:
+    dict(add, mul, div, sqrt, exp, log, pow, atan, cbrt) and the outputs."""
+    lib, _ = load("count")
+    pts = _f64(pts)
+    npts = pts.shape[1]
+    x, w = gauss_legendre(nquad)
+    out = np.zeros((20, npts))
+    counts = np.zeros(9, dtype=np.int64)
+    lib.orc_count_m1_opacities(npts, nquad, x, w, nquad, x, w, pts,
+                               np.ascontiguousarray(flags, np.int32),
+                               np.ascontiguousarray(params, np.int32), int(stim), out, counts)
+    names = ["add", "mul", "div", "sqrt", "exp", "log", "pow", "atan", "cbrt"]
+    return dict(zip(names, (int(c) for c in counts))), out

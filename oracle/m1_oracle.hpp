@@ -1,0 +1,1445 @@
+// oracle/m1_oracle.hpp -- CPU restatement (FP64) of the bns_nurates gray M1
+// opacity path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+// kernels of bns_nurates_amd.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may build, load or call it, and only as the
+// checker / the timed CPU baseline -- never as the product path.
+//
+// Every function restates the algorithm of the reference file:line cited in
+// its comment, with the same operation order, so that with the same compiler
+// flags (-O2 -ffp-contract=off, no FMA) it reproduces the reference bit for
+// bit.  It is written "reuse-aware": every distinct Fermi-Dirac integral and
+// distribution value is evaluated once per point (node tables, shared
+// argument bundles), which changes no bits because each reused value is
+// produced by the same arithmetic the reference repeats.
+//
+// All functions are templates over the real type R so that the same code is
+// instantiated with a counting type (oracle/flop_count.cpp) to tally the
+// algorithmic FLOPs per point used for roofline figures.
+//
+// Parity status per function (how each is pinned; see tests/):
+//   GaussLegendre      closed forms n=2,3,4 (reference test), numpy leggauss,
+//                      sum(w)=1, exactness on x^k  + reference vectors
+//   fdi<k>             mpmath polylog closed form F_k = -k! Li_{k+1}(-e^eta),
+//                      F_k(0) = (1-2^-k) k! zeta(k+1), reference vectors
+//   pair kernels       18 values of the reference's python comparison,
+//                      detailed balance abs/em = e^{(w+w')/T}, reference vectors
+//   brem HR98          reference known-answer test (2 values, 1e-6)
+//   brem BRT06         detailed balance e^{2x}; scipy K1 (~1e-7 fit);
+//                      reference vectors
+//   NES/NPS            reference vectors over all three branches; in/out
+//                      detailed balance
+//   beta / iso         reference CCSN tables (6 s.f.), detailed balance,
+//                      reference vectors
+//   distribution       reference vectors over all y-branches; chi in {1/3,1}
+//   gray assembly      reference end-to-end vectors (n=4..10), Kirchhoff
+//                      invariant with equilibrium distributions, flag
+//                      additivity
+#ifndef BNS_NURATES_AMD_ORACLE_M1_ORACLE_HPP_
+#define BNS_NURATES_AMD_ORACLE_M1_ORACLE_HPP_
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Math shims: overloaded for double here and for the counting type in
+// oracle/flop_count.cpp.
+// ---------------------------------------------------------------------------
+inline double rexp(double x) { return std::exp(x); }
+inline double rlog(double x) { return std::log(x); }
+inline double rpow(double x, double y) { return std::pow(x, y); }
+inline double rsqrt(double x) { return std::sqrt(x); }
+inline double ratan(double x) { return std::atan(x); }
+inline double rcbrt(double x) { return std::cbrt(x); }
+inline double rfabs(double x) { return std::fabs(x); }
+inline bool rsignbit(double x) { return std::signbit(x); }
+inline double rval(double x) { return x; }
+
+// ---------------------------------------------------------------------------
+// Physical constants (constants.hpp:31-244).  Fundamental constants are
+// doubles; the derived ones are evaluated from them in double with the same
+// expression shapes, so every bit agrees with the reference.
+// ---------------------------------------------------------------------------
+namespace k {
+constexpr double MeV = 1.6021766341182763e+8;   // constants.hpp:32
+constexpr double Me = 0.51099895069;            // :44
+constexpr double Mn = 939.56542194;             // :50
+constexpr double Mp = 938.27208943;             // :53
+constexpr double Q = 1.29333251;                // :56
+constexpr double Mu = 1.66053906892e-24;        // :59
+constexpr double Mb = 1.6737747126723517e-24;   // :62
+constexpr double Pi = 3.14159265358979323846264338328;  // :65
+constexpr double H = 4.135667696e-21;           // :68
+constexpr double Hbar = 0.5 * H / Pi;           // :71
+constexpr double Clight = 2.99792458e+17;       // :74
+constexpr double Gf0 = 1.1663787e-11;           // :80
+constexpr double Gf = Gf0 * ((Hbar * Clight) * (Hbar * Clight) * (Hbar * Clight));  // :81
+constexpr double SaturationN = 0.15e+18;        // :85
+constexpr double Ga = 1.23, Gv = 1., Gs = 0., SinThW2 = 0.2325;  // :92-95
+constexpr double Hnv = -0.5, Hna = -0.5 * Ga;                    // :98-99
+constexpr double Hpv = 0.5 - 2. * SinThW2, Hpa = 0.5 * Ga;       // :100-101
+constexpr double BetaConst =                                      // :109
+    ((Clight * Clight * Clight) * ((Gf0 * Hbar) * (Gf0 * Hbar))) *
+    ((Gv * Gv) + 3. * (Ga * Ga)) / Pi;
+constexpr double IsoEF = 4.785390000313652 * (Hbar * Hbar) / Mb * MeV;  // :120
+constexpr double IsoConst =                                              // :123
+    2. * Pi * (Gf * Gf) / Hbar / ((H * Clight) * (H * Clight) * (H * Clight));
+constexpr double IsoC0p = IsoConst * ((Hpv * Hpv) + 3. * (Hpa * Hpa));  // :132
+constexpr double IsoC1p = IsoConst * ((Hpv * Hpv) - (Hpa * Hpa));       // :134
+constexpr double IsoC0n = IsoConst * ((Hnv * Hnv) + 3. * (Hna * Hna));  // :136
+constexpr double IsoC1n = IsoConst * ((Hnv * Hnv) - (Hna * Hna));       // :138
+constexpr double BremXmin = 1.0e-10, BremYmin = 1.0e-10, BremEtamin = 1.0e-10;  // :145-147
+constexpr double BremAux1 = 0.25 * (H * H) * MeV / (Pi * Pi);  // :149
+constexpr double BremCa = -0.63;                               // :153
+constexpr double BremConst = (BremCa * BremCa) * (Gf * Gf) / Hbar;  // :155
+constexpr double BremC4BRT06 = 0.1409912109375;                    // :159
+constexpr double GSqr = 1.55e-12;                                  // :167
+constexpr double PairPhiC = GSqr / Pi;                             // :169
+constexpr double PairA1E = 1. + 2. * SinThW2, PairA2E = 2. * SinThW2;   // :172-174
+constexpr double PairA1X = -1. + 2. * SinThW2, PairA2X = 2. * SinThW2;  // :176-178
+constexpr double NEPSConst =                                            // :190
+    2. * (Gf0 * Gf0) * Clight * ((Hbar * Clight) * (Hbar * Clight)) / (3. * Pi);
+constexpr double NEPSBPlus = (2. * SinThW2 + 1.) * (2. * SinThW2 + 1.);   // :194
+constexpr double NEPSBMinus = (2. * SinThW2 - 1.) * (2. * SinThW2 - 1.);  // :195
+constexpr double NEPSBZero = 4. * (SinThW2 * SinThW2);                  // :196
+constexpr double ExpUpp = 700., ExpLow = -700.;                          // :204-207
+constexpr double HC6FourPiSq =                                           // :209
+    ((H * Clight) * (H * Clight) * (H * Clight) * (H * Clight) * (H * Clight) *
+     (H * Clight)) /
+    (16. * (Pi * Pi));
+constexpr double WMEScale = MeV / (Mb * (Clight * Clight));  // :213
+constexpr double FourPiHc3 = (4. * Pi) / ((H * Clight) * (H * Clight) * (H * Clight));  // :216
+constexpr double FourPiHc3Sqr = FourPiHc3 * FourPiHc3;                  // :218
+constexpr double MnGrams = Mn * MeV / (Clight * Clight);                // :222
+constexpr double MpGrams = Mp * MeV / (Clight * Clight);                // :226
+constexpr double MAvgGrams = MeV / (Clight * Clight) * 938.918532994116257276238085018813;  // :230
+constexpr double SqrtPi = 1.77245385090551602729816748334106;      // :234
+constexpr double PiSquared = 9.86960440108935861883449099987572;   // :236
+constexpr double FourPiSquared = 39.4784176043574344753379639995029;  // :238
+constexpr double Pi2OneEighth = 1.15383506784998943054096521314981;   // :240
+constexpr double PiHalfToFiveHalves = 3.09242868139914350627854469835251;  // :242
+constexpr double ThresholdN = 1e-21, ThresholdJ = 1e-25;  // m1_opacities.hpp:19-20
+constexpr double FdiE2 = 7.38905609893065023;             // functions.hpp:23 (e^2)
+}  // namespace k
+
+enum { NUE = 0, ANUE = 1, NUX = 2, ANUX = 3, NSP = 4 };
+constexpr int partner(int s) { return s ^ 1; }
+
+// Layout of the SoA point block, the optional distribution block and the
+// output block of the batched entry point (one field = npts doubles).
+enum {
+  P_NB = 0, P_T, P_YP, P_YN, P_MUP, P_MUN, P_MUE, P_DU, P_DMEFF,
+  P_N0, P_J0 = P_N0 + 4, P_CHI0 = P_J0 + 4, P_NFIELDS = P_CHI0 + 4  // 21
+};
+enum {
+  D_WT = 0, D_TT = 4, D_ETAT = 8, D_WF = 12, D_TF = 16, D_CF = 20, D_BF = 24,
+  D_NFIELDS = 28
+};
+enum {
+  O_ETA0 = 0, O_KAPPA0A = 4, O_ETA = 8, O_KAPPAA = 12, O_KAPPAS = 16,
+  O_NFIELDS = 20
+};
+
+struct Flags {  // OpacityFlags (bns_nurates.hpp:418-426)
+  int use_abs_em, use_pair, use_brem, use_inelastic_scatt, use_iso;
+};
+struct Params {  // OpacityParams (bns_nurates.hpp:338-352)
+  int use_dU, use_dm_eff, use_WM_ab, use_WM_sc, use_decay, brem_impl,
+      use_NN_medium_corr, neglect_blocking;
+};
+enum { BREM_HR98 = 0, BREM_BRT06 = 1, BREM_GP19 = 2 };
+
+template <class R>
+struct Eos {  // the MyEOSParams fields the path reads (bns_nurates.hpp:289-306)
+  R nb, T, yp, yn, mu_p, mu_n, mu_e, dU, dm_eff;
+};
+template <class R>
+struct Distr {  // NuDistributionParams (bns_nurates.hpp:387-401)
+  R w_t[4], temp_t[4], eta_t[4], w_f[4], temp_f[4], c_f[4], beta_f[4];
+};
+template <class R>
+struct M1 {  // M1Quantities without the unused flux (bns_nurates.hpp:409-416)
+  R n[4], J[4], chi[4];
+};
+template <class R>
+struct KOut {  // MyKernelOutput (bns_nurates.hpp:259-263)
+  R em[4], abs[4];
+};
+
+// ---------------------------------------------------------------------------
+// Special functions (functions.hpp)
+// ---------------------------------------------------------------------------
+
+// SafeExp, functions.hpp:66-75
+template <class R>
+inline R safe_exp(R x) {
+  R lo = (x > R(k::ExpLow)) ? x : R(k::ExpLow);
+  R c = (lo < R(k::ExpUpp)) ? lo : R(k::ExpUpp);
+  return rexp(c);
+}
+
+// FermiDistr, functions.hpp:4376-4395
+template <class R>
+inline R fermi_distr(R e, R temp, R mu) {
+  R arg = (e - mu) / temp;
+  if (arg > R(0.)) {
+    R t = safe_exp(-arg);
+    return t / (t + R(1.));
+  }
+  R t = safe_exp(arg);
+  return R(1.) / (t + R(1.));
+}
+
+// Coefficients of the piecewise minimax rational approximations of the
+// complete Fermi-Dirac integrals F_k, k = 0..5 (Fukushima 2015, as tabulated
+// in functions.hpp:1558-1574, 1803-1820, 2059-2074, 2301-2315, 2543-2557,
+// 2786-2801).  Stored in evaluation (Horner) order with the signs folded in:
+//   branch x < -2:  t = e^x e^2,  F = e^x (a0 - e^x Pa(t) / Qa(t))
+//   branch x <= 0:  s = -x/2, t = 1 - s,  F = Pb(t) / Qb(s)
+// Pa, Pb list the coefficients from the highest power down; Qa, Qb are monic
+// up to the sign `lead` of their highest power.
+struct FdiCoef {
+  int na;        // number of Pa coefficients
+  double a0;     // leading constant of the x < -2 branch
+  double pa[5];  // Pa, highest power first
+  int nqa;       // Qa non-leading coefficients
+  double qa[4];  // Qa below the monic t^nqa term, highest power first
+  int nb;
+  double pb[9];
+  int nqb;
+  double qb_lead;
+  double qb[8];
+};
+
+// functions.hpp:1558-1574 (F_0)
+constexpr FdiCoef kFdi0 = {
+    5, 1.,
+    {0.00200096064827815813, 6.9167792879948140, 357.623326425354522,
+     5222.0667923565138, 22696.2126132366633},
+    4, {71.072178562726798, 1611.36476693109675, 14539.5980679273792,
+        45392.4252264733267},
+    9,
+    {0.000915454570009894267, -0.0734798777056723512, -0.450072959113928254,
+     -0.441998676614933572, 3.78181326142271599, 10.5181677709577503,
+     0.377783268730614356, 23.7193942338278703, 159.601717762460980},
+    7, 1.,
+    {8.1762315442738975, 35.456591489081173, 96.77898293084927,
+     206.21640678892182, 310.2713981221035, 315.2592651624449,
+     284.26032127745967}};
+// functions.hpp:1803-1820 (F_1); B[8] and the s^8 term enter with a minus sign
+constexpr FdiCoef kFdi1 = {
+    5, 1.,
+    {0.00397641173774375092, 5.9897442965804548, 322.901386168881348,
+     4915.92700908746777, 22189.1070807945062},
+    4, {88.376214553692756, 2389.06277237306633, 25002.3197546553836,
+        88756.428323178025},
+    9,
+    {-0.0792966701498222697, -0.403173132925886253, 0.199701180197912643,
+     7.09659390228556164, 20.1369115558099802, 17.9918985363509694,
+     56.6537141912783024, 251.392824471576922, 145.488167182330098},
+    8, -1.,
+    {-7.1995819520154718, -25.678454878692950, -53.72117554363975,
+     -61.57766112137513, 27.2746245830016, 252.1367051536344,
+     374.1806357435014, 606.0757707716040}};
+// functions.hpp:2059-2074 (F_2)
+constexpr FdiCoef kFdi2 = {
+    4, 2.,
+    {0.0161890243763741414, 8.5861610217850095, 273.085756700981399,
+     1914.06748184935743, 0.},
+    3, {72.929152915475392, 1399.35442210906621, 7656.2699273974454, 0.},
+    9,
+    {-0.326188299771397236, -2.14077421411719935, -2.68393549333878715,
+     24.0376482128898634, 112.951038040682055, 172.881855215582924,
+     222.606134197895041, 1299.85460914884154, 2711.49678259128843},
+    8, -1.,
+    {-3.9142702096919080, 8.395768655115050, 122.54595216479181,
+     531.62378035996132, 1428.0589853413436, 2541.7823512372631,
+     3038.7689794575778, 2517.1726659917047}};
+// functions.hpp:2301-2315 (F_3)
+constexpr FdiCoef kFdi3 = {
+    4, 6.,
+    {0.0410982603688952131, 19.0856927562699544, 664.28706260743472,
+     5121.6401850302408, 0.},
+    3, {92.376788603062645, 2136.54222460571183, 13657.7071600806539, 0.},
+    8,
+    {1.59407954898394322, 19.9559488532742796, 113.373708671587772,
+     354.630774329461644, 653.359212389160499, 1260.13125873282465,
+     4323.07526636309661, 7881.24597452900838, 0.},
+    7, 1.,
+    {16.468882811659000, 112.60906419590854, 459.86413596901097,
+     1259.0724833462608, 2393.9995533270879, 2972.7443644211129,
+     2570.7250703533430, 0.}};
+// functions.hpp:2543-2557 (F_4)
+constexpr FdiCoef kFdi4 = {
+    4, 24.,
+    {0.121876197098273914, 54.659507299984584, 2120.56302902849207,
+     18247.2542465629199, 0.},
+    3, {117.071576489684022, 3261.01909656925521, 24329.6723287505678, 0.},
+    8,
+    {6.67374311450268357, 77.4108960876508190, 418.662172475927160,
+     1268.40017257978070, 2393.94262931609398, 5431.08357152245897,
+     19031.4783357798975, 33093.9102025608137, 0.},
+    7, 1.,
+    {15.216887271751039, 105.86020755838874, 448.99020896813485,
+     1278.0109577275445, 2500.5977847175497, 3236.2237702166948,
+     2645.4885670047153, 0.}};
+// functions.hpp:2786-2801 (F_5)
+constexpr FdiCoef kFdi5 = {
+    4, 120.,
+    {0.413800735538960261, 190.753830813273698, 8368.4990332049831,
+     81190.938912603315, 0.},
+    3, {148.484432990224213, 4977.68099709243407, 43301.8340867217726, 0.},
+    8,
+    {23.6526046298891619, 272.164427980501432, 1461.45267097859337,
+     4445.76333033698006, 9274.54751848368696, 26923.5693307648389,
+     96307.2005742063042, 159651.547840031111, 0.},
+    7, 1.,
+    {13.491911254479298, 91.84661231161838, 397.7596246691212,
+     1176.7202478443275, 2403.0532924519756, 3244.5527999477567,
+     2522.7839609396783, 0.}};
+
+constexpr const FdiCoef* kFdiTab[6] = {&kFdi0, &kFdi1, &kFdi2,
+                                       &kFdi3, &kFdi4, &kFdi5};
+
+// F_k(x) for x = -|y| <= 0 (the part shared by all orders), given e^x when
+// x < -2.  Horner forms reproduce the nesting of functions.hpp exactly:
+// c_m + t*(c_{m-1}... written from the innermost term outwards.
+template <class R>
+inline R fdi_neg_exp_branch(const FdiCoef& c, R ex) {
+  R t = ex * R(k::FdiE2);
+  R p = R(c.pa[0]);
+  for (int i = 1; i < c.na; ++i) p = R(c.pa[i]) + t * p;
+  R q = t;  // monic: innermost term is (q_{nqa-1} + t)
+  q = R(c.qa[0]) + q;
+  for (int i = 1; i < c.nqa; ++i) q = R(c.qa[i]) + t * q;
+  return ex * (R(c.a0) - ex * p / q);
+}
+template <class R>
+inline R fdi_neg_rat_branch(const FdiCoef& c, R x) {
+  R s = -R(0.5) * x;
+  R t = R(1.) - s;
+  R p = R(c.pb[0]);
+  for (int i = 1; i < c.nb; ++i) p = R(c.pb[i]) + t * p;
+  R q = R(c.qb_lead);
+  for (int i = 0; i < c.nqb; ++i) q = R(c.qb[i]) + s * q;
+  return p / q;
+}
+
+// Reflection to y > 0 (exact polynomial parts), functions.hpp:1606-1609,
+// 1854-1857, 2106-2109, 2343-2347, 2585-2589, 2829-2834.
+template <class R>
+inline R fdi_reflect(int kord, R fd, R y) {
+  switch (kord) {
+    case 0: return fd + y;
+    case 1: return -fd + R(1.64493406684822644) + R(0.5) * y * y;
+    case 2: return fd + y * (R(3.28986813369645287) + R(0.333333333333333333) * y * y);
+    case 3: {
+      R y2 = y * y;
+      return -fd + R(11.3643939539669510) + y2 * (R(4.93480220054467931) + y2 * R(0.25));
+    }
+    case 4: {
+      R y2 = y * y;
+      return fd + y * (R(45.4575758158678040) + y2 * (R(6.57973626739290575) + y2 * R(0.2)));
+    }
+    default: {
+      R y2 = y * y;
+      return -fd + R(236.532261911384425) +
+             y2 * (R(113.643939539669510) + y2 * (R(8.22467033424113218) + y2 * R(0.166666666666666667)));
+    }
+  }
+}
+
+// Complete Fermi-Dirac integral F_k(y), k = 0..5 (FDI_0 .. FDI_p5,
+// functions.hpp:1551-1612, 1796-1860, 2052-2112, 2294-2350, 2536-2592,
+// 2779-2837).
+template <class R>
+inline R fdi(int kord, R y) {
+  const FdiCoef& c = *kFdiTab[kord];
+  R x = -rfabs(y);
+  R fd = R(0.);
+  if (x < R(-2.)) {
+    fd = fdi_neg_exp_branch(c, rexp(x));
+  } else if (x <= R(0.)) {
+    fd = fdi_neg_rat_branch(c, x);
+  }
+  if (y > R(0.)) fd = fdi_reflect(kord, fd, y);
+  return fd;
+}
+
+// A bundle of orders k = klo..5 at one argument: one exponential shared by
+// all orders (the values are bit-identical to separate fdi() calls).
+template <class R>
+inline void fdi_bundle(int klo, R y, R* out /* out[k] for k=klo..5 */) {
+  R x = -rfabs(y);
+  bool e = x < R(-2.);
+  bool r = !e && (x <= R(0.));
+  R ex = e ? rexp(x) : R(0.);
+  for (int kk = klo; kk <= 5; ++kk) {
+    const FdiCoef& c = *kFdiTab[kk];
+    R fd = R(0.);
+    if (e) fd = fdi_neg_exp_branch(c, ex);
+    else if (r) fd = fdi_neg_rat_branch(c, x);
+    if (y > R(0.)) fd = fdi_reflect(kk, fd, y);
+    out[kk] = fd;
+  }
+}
+
+// bessi1, functions.hpp:99-136 (polynomial fit, Abramowitz & Stegun 9.8.3-4)
+template <class R>
+inline R bessi1(R x) {
+  R ax, ans, y;
+  if ((ax = rfabs(x)) < R(3.75)) {
+    y = x / R(3.75), y = y * y;
+    ans = ax * (R(0.5) + y * (R(0.87890594) + y * (R(0.51498869) + y * (R(0.15084934) +
+                 y * (R(0.2658733e-1) + y * (R(0.301532e-2) + y * R(0.32411e-3)))))));
+  } else {
+    y = R(3.75) / ax;
+    ans = R(0.2282967e-1) + y * (R(-0.2895312e-1) + y * (R(0.1787654e-1) - y * R(0.420059e-2)));
+    ans = R(0.39894228) + y * (R(-0.3988024e-1) + y * (R(-0.362018e-2) + y * (R(0.163801e-2) +
+          y * (R(-0.1031555e-1) + y * ans))));
+    ans *= (rexp(ax) / rsqrt(ax));
+  }
+  return x < R(0.) ? -ans : ans;
+}
+
+// bessk1, functions.hpp:141-180 (polynomial fit, A&S 9.8.7-8)
+template <class R>
+inline R bessk1(R x) {
+  R y, ans;
+  if (x <= R(2.)) {
+    y = x * x / R(4.);
+    ans = (rlog(x / R(2.)) * bessi1(x)) +
+          (R(1.0) / x) * (R(1.0) + y * (R(0.15443144) + y * (R(-0.67278579) +
+           y * (R(-0.18156897) + y * (R(-0.1919402e-1) + y * (R(-0.110404e-2) + y * (R(-0.4686e-4))))))));
+  } else {
+    y = R(2.) / x;
+    ans = (rexp(-x) / rsqrt(x)) *
+          (R(1.25331414) + y * (R(0.23498619) + y * (R(-0.3655620e-1) + y * (R(0.1504268e-1) +
+           y * (R(-0.780353e-2) + y * (R(0.325614e-2) + y * (R(-0.68245e-3))))))));
+  }
+  return ans;
+}
+
+// NEPSExpFunc, functions.hpp:4401-4412
+template <class R>
+inline R neps_exp_func(R x) {
+  R ex = safe_exp(-rfabs(x));
+  int is_neg = rsignbit(x) ? 1 : 0;
+  return (R(double(is_neg)) - R(double(!is_neg)) * ex) /
+         (R(1.) - ex + R(double(rval(x) == 0.)));
+}
+
+// GammaStirling, functions.hpp:4460-4470
+template <class R>
+inline R gamma_stirling(R x) {
+  return rsqrt(R(6.283185307179586) / x) * rpow(x / R(2.718281828459045235360287471352), x);
+}
+
+// ---------------------------------------------------------------------------
+// Weak magnetism (weak_magnetism.hpp)
+// ---------------------------------------------------------------------------
+// NucFrmFac, weak_magnetism.hpp:45-104
+template <class R>
+inline void nuc_form_factors(R E, int reac, R& cv, R& ca, R& F2) {
+  const double lamp = 1.793, lamn = -1.913;
+  R e = E * R(k::WMEScale);
+  R tau = R(0.5) * (e * e) / (R(1.) + e);
+  R eta = R(1.) / (R(1.) + R(5.6) * tau);
+  R G = R(1.) / rpow(R(1.) + R(4.97) * tau, R(2.));
+  R Fp1 = (R(1.) + tau * (R(1.) + R(lamp))) * G / (R(1.) + tau);
+  R Fp2 = R(lamp) * G / (R(1.) + tau);
+  R Fn1 = tau * R(lamn) * (R(1.) - eta) * G / (R(1.) + tau);
+  R Fn2 = R(lamn) * (R(1.) + tau * eta) * G / (R(1.) + tau);
+  R c3t = R(1.) + R(3.53) * tau;
+  if (reac == 1) {
+    cv = (R(0.5) - R(2.) * R(k::SinThW2)) * Fp1 - R(0.5) * Fn1;
+    ca = R(0.5) * (R(k::Ga) - R(k::Gs)) / (c3t * c3t);
+    F2 = (R(0.5) - R(2.) * R(k::SinThW2)) * Fp2 - R(0.5) * Fn2;
+  } else if (reac == 2) {
+    cv = (R(0.5) - R(2.) * R(k::SinThW2)) * Fn1 - R(0.5) * Fp1;
+    ca = -R(0.5) * (R(k::Ga) + R(k::Gs)) / (c3t * c3t);
+    F2 = (R(0.5) - R(2.) * R(k::SinThW2)) * Fn2 - R(0.5) * Fp2;
+  } else {
+    cv = Fp1 - Fn1;
+    ca = R(k::Ga) / (c3t * c3t);
+    F2 = Fp2 - Fn2;
+  }
+}
+
+// WMAbsEm, weak_magnetism.hpp:123-160
+template <class R>
+inline void wm_abs_em(R omega, R& Rnu, R& Rbar) {
+  R cv, ca, F2;
+  nuc_form_factors(omega, 3, cv, ca, F2);
+  R e = omega * R(k::WMEScale);
+  R a = R(1.) + R(4. / 3.) * e;
+  R tmp1 = (cv * cv) * (R(1.) + R(4.) * e + R(16. / 3.) * (e * e)) +
+           R(3.) * ca * ca * (a * a) + R(8. / 3.) * cv * F2 * (e * e) +
+           R(5. / 3.) * (e * e) * (R(1.) + R(2. / 5.) * e) * (F2 * F2);
+  R tmp2 = R(4.) * (cv + F2) * ca * e * (R(1.) + R(4. / 3.) * e);
+  R b = R(1.) + R(2.) * e;
+  R tmp3 = (R(k::Gv) * R(k::Gv) + R(3.) * (R(k::Ga) * R(k::Ga))) * (b * b * b);
+  Rnu = (tmp1 + tmp2) / tmp3;
+  Rbar = (tmp1 - tmp2) / tmp3;
+}
+
+// WMScatt, weak_magnetism.hpp:165-210
+template <class R>
+inline void wm_scatt(R omega, int reac, R& R0, R& R1) {
+  R cv, ca, F2;
+  nuc_form_factors(omega, reac, cv, ca, F2);
+  R h0, h1;
+  if (reac == 1) {
+    h0 = R(k::Hpv) * R(k::Hpv) + R(3.) * (R(k::Hpa) * R(k::Hpa));
+    h1 = R(k::Hpv) * R(k::Hpv) - R(k::Hpa) * R(k::Hpa);
+  } else {
+    h0 = R(k::Hnv) * R(k::Hnv) + R(3.) * (R(k::Hna) * R(k::Hna));
+    h1 = R(k::Hnv) * R(k::Hnv) - R(k::Hna) * R(k::Hna);
+  }
+  R e = omega * R(k::WMEScale);
+  R eF = e * F2;
+  R0 = (cv * cv + R(3.) * (ca * ca) + R(1.5) * (eF * eF) + R(2.) * (e * e) * cv * F2) / h0;
+  R1 = (cv * cv - ca * ca - R(2.) * (eF * eF) - R(4.) * (e * e) * cv * F2) / h1;
+}
+
+// ---------------------------------------------------------------------------
+// Beta processes (formulas_beta_iso.hpp)
+// ---------------------------------------------------------------------------
+// EtaNNAbs, formulas_beta_iso.hpp:64-97
+template <class R>
+inline R eta_nn_abs(R n_in, R n_out, R mu_hat, R T) {
+  if (rfabs(mu_hat) < R(1.0e-02)) return n_in;
+  R etanp = (n_out - n_in) / (safe_exp(-mu_hat / T) - R(1.));
+  if (etanp < R(0.)) return n_in;
+  return etanp;
+}
+
+// AbsOpacitySingleLep (electrons) + AbsOpacity, formulas_beta_iso.hpp:138-319.
+// Returns em/abs for nue (index 0) and anue (index 1).
+template <class R>
+inline void abs_opacity(R omega, const Params& p, const Eos<R>& eos, R em[2], R ab[2]) {
+  const R m = R(k::Me);
+  const R m2 = m * m;
+  const R mu_l = eos.mu_e;
+  R nn = eos.nb * eos.yn, np = eos.nb * eos.yp;
+  R dU = p.use_dU ? eos.dU : R(0.);
+  R dQ = p.use_dm_eff ? eos.dm_eff : R(k::Q);
+  R mu_hat = eos.mu_n - eos.mu_p - dQ;
+  R Qp = dQ + dU;
+  R mu_np = mu_hat - dU;
+  R etanp = eta_nn_abs(nn, np, mu_np, eos.T);
+  R etapn = eta_nn_abs(np, nn, -mu_np, eos.T);
+  R Rw = R(1.), Rb = R(1.);
+  if (p.use_WM_ab) wm_abs_em(omega, Rw, Rb);
+
+  // neutrino: nue + n <-> e- + p
+  R Ee = omega + Qp, Ep = -Ee;
+  R Ee2 = Ee * Ee, Ep2 = Ep * Ep;
+  R fde = fermi_distr(Ee, eos.T, mu_l);
+  R fdp = fermi_distr(Ep, eos.T, -mu_l);
+  R cap = R(0.), dec = R(0.);
+  if (Ee - m >= R(0.)) cap = Ee2 * rsqrt(R(1.) - m2 / Ee2) * Rw;
+  if (p.use_decay && (Ep - m >= R(0.))) dec = Ep2 * rsqrt(R(1.) - m2 / Ep2);
+  em[0] = R(k::BetaConst) * etapn * (cap * fde + dec * (R(1.) - fdp));
+  ab[0] = em[0] * safe_exp((omega - (eos.mu_p + mu_l - eos.mu_n)) / eos.T);
+
+  // antineutrino: anue + p <-> e+ + n
+  cap = R(0.);
+  dec = R(0.);
+  Ep = omega - Qp;
+  Ee = -Ep;
+  Ee2 = Ee * Ee;
+  Ep2 = Ep * Ep;
+  fde = fermi_distr(Ee, eos.T, mu_l);
+  fdp = fermi_distr(Ep, eos.T, -mu_l);
+  if (Ep - m >= R(0.)) cap = Ep2 * rsqrt(R(1.) - m2 / Ep2) * Rb;
+  if (p.use_decay && (Ee - m >= R(0.))) dec = Ee2 * rsqrt(R(1.) - m2 / Ee2);
+  em[1] = R(k::BetaConst) * etanp * (cap * fdp + dec * (R(1.) - fde));
+  ab[1] = em[1] * safe_exp((omega - (eos.mu_n - eos.mu_p - mu_l)) / eos.T);
+}
+
+// ---------------------------------------------------------------------------
+// Isoenergetic scattering (formulas_beta_iso.hpp:371-507)
+// ---------------------------------------------------------------------------
+template <class R>
+inline R eta_nn_sc(R nb, R temp, R yN) {
+  R nN = yN * nb;
+  if (nN <= R(0.)) return R(0.);
+  R eF = R(k::IsoEF) * rpow(nN, R(2. / 3.));
+  R aux = R(1.5) * temp / eF;
+  return nN * aux / rsqrt(R(1.) + aux * aux);
+}
+template <class R>
+inline R iso_scatt_total(R omega, const Params& p, const Eos<R>& eos) {
+  R tot[2];
+  for (int reac = 1; reac <= 2; ++reac) {
+    R yN = reac == 1 ? eos.yp : eos.yn;
+    R etaNN = eta_nn_sc(eos.nb, eos.T, yN);
+    R R0 = R(1.), R1 = R(1.);
+    if (p.use_WM_sc) wm_scatt(omega, reac, R0, R1);
+    R leg0 = (reac == 1 ? R(k::IsoC0p) : R(k::IsoC0n)) * R0;
+    R leg1 = (reac == 1 ? R(k::IsoC1p) : R(k::IsoC1n)) * R1;
+    tot[reac - 1] = etaNN * (leg0 - leg1 / R(3.));
+  }
+  return tot[0] + tot[1];
+}
+
+// ---------------------------------------------------------------------------
+// Pair kernel (kernel_pair.hpp).  F[k][m] are the node/cell Fermi integrals
+// F_k at the arguments m: 0 eta-y, 1 eta-z, 2 eta+y, 3 eta+z, 4 eta,
+// 5 eta-y-z, 6 eta+y+z.
+// ---------------------------------------------------------------------------
+enum { A_EMY = 0, A_EMZ, A_EPY, A_EPZ, A_E, A_EMYZ, A_EPYZ };
+
+// PairPsi, kernel_pair.hpp:16-120 (cutoff test done by the caller)
+template <class R>
+inline void pair_psi(R y, R z, const R F[6][7], R psi[2]) {
+  const R F1emy = F[1][A_EMY], F1emz = F[1][A_EMZ], F1epy = F[1][A_EPY], F1epz = F[1][A_EPZ];
+  const R F2emy = F[2][A_EMY], F2emz = F[2][A_EMZ], F2epy = F[2][A_EPY], F2epz = F[2][A_EPZ];
+  const R F3e = F[3][A_E], F3emy = F[3][A_EMY], F3emz = F[3][A_EMZ], F3epy = F[3][A_EPY],
+          F3epz = F[3][A_EPZ], F3emyz = F[3][A_EMYZ], F3epyz = F[3][A_EPYZ];
+  const R F4e = F[4][A_E], F4emy = F[4][A_EMY], F4emz = F[4][A_EMZ], F4epy = F[4][A_EPY],
+          F4epz = F[4][A_EPZ], F4emyz = F[4][A_EMYZ], F4epyz = F[4][A_EPYZ];
+  const R F5emy = F[5][A_EMY], F5emz = F[5][A_EMZ], F5epy = F[5][A_EPY], F5epz = F[5][A_EPZ],
+          F5emyz = F[5][A_EMYZ], F5epyz = F[5][A_EPYZ];
+  const R c4 = R(4.), c20 = R(20.), c40 = R(40.), c80 = R(80.), c120 = R(120.);
+  R x0 = c20 * F4emy;
+  R x1 = c20 * F4epz;
+  R x2 = c120 * F2emy - c120 * F2epz;
+  R x3 = -c40 * F3emy + c40 * F3epz;
+  R x4 = c40 * F3e;
+  R x5 = c40 * F3emyz - x4;
+  R x6 = -c20 * F4e;
+  R x7 = c20 * F4emyz + x6;
+  R x8 = -c40 * F3epyz + x4;
+  R x9 = c20 * F4epyz + x6;
+  R x10 = -c4 * F5emy + c4 * F5emyz - c4 * F5emz + c4 * F5epy - c4 * F5epyz + c4 * F5epz;
+  R x11 = c20 * F4epy;
+  R x12 = c20 * F4emz;
+  R x13 = -c40 * F3emz + c40 * F3epy;
+  R x14 = c120 * F2emz - c120 * F2epy;
+  R aux = R(15.) * ((y * z) * (y * z));
+  psi[0] = x10 +
+           y * (-x0 + x1 + x7 +
+                y * (x3 + x5 + z * (x2 + z * (-c120 * F1emy + c120 * F1epz))) +
+                z * (c80 * F3emy - c80 * F3epz - x2 * z)) +
+           z * (x0 - x1 + x9 + z * (x3 + x8));
+  psi[1] = x10 + y * (-x11 + x12 + x9 + y * (x13 + x8)) +
+           z * (x11 - x12 + x7 + y * (c80 * F3emz - c80 * F3epy - x14 * y) +
+                z * (x13 + x5 + y * (x14 + y * (-c120 * F1emz + c120 * F1epy))));
+  psi[0] /= aux;
+  psi[1] /= aux;
+}
+
+// PairPhi + PairKernels (forward), kernel_pair.hpp:135-198.  y, z are
+// omega/T and omega'/T; F as for pair_psi.
+template <class R>
+inline void pair_kernel_fwd(R omega, R omega_p, R T, R eta, R y, R z,
+                            const R F[6][7], KOut<R>& out) {
+  R aux = R(k::PairPhiC) * (T * T) / (R(1.) - safe_exp(y + z));
+  R psi[2] = {R(0.), R(0.)};
+  if (!(eta - y - z > R(200.))) pair_psi(y, z, F, psi);
+  const R a1e = R(k::PairA1E) * R(k::PairA1E), a2e = R(k::PairA2E) * R(k::PairA2E);
+  const R a1x = R(k::PairA1X) * R(k::PairA1X), a2x = R(k::PairA2X) * R(k::PairA2X);
+  R phi[4];
+  phi[0] = (a1e * psi[0] + a2e * psi[1]) * aux;
+  phi[1] = (a1e * psi[1] + a2e * psi[0]) * aux;
+  phi[2] = (a1x * psi[0] + a2x * psi[1]) * aux;
+  phi[3] = (a1x * psi[1] + a2x * psi[0]) * aux;
+  R db = safe_exp((omega + omega_p) / T);
+  for (int s = 0; s < 4; ++s) {
+    out.em[s] = R(0.5) * phi[s];
+    out.abs[s] = db * out.em[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NN bremsstrahlung
+// ---------------------------------------------------------------------------
+// QBrem_BRT06 + BremKernelsBRT06, kernel_brem_BRT06.hpp:24-69
+template <class R>
+inline void brem_brt06(R omega, R omega_p, const Eos<R>& eos, R& s_em, R& s_abs) {
+  R T = eos.T;
+  R x = R(0.5) * (omega + omega_p) / T;
+  R rho = eos.nb * R(k::Mb);
+  R q = R(2.0778e+02) * R(0.5) * R(k::MeV) *
+        (eos.yn * eos.yn + eos.yp * eos.yp + R(28. / 3.) * eos.yn * eos.yp) * (rho * rho) *
+        rpow(T, R(5.5));
+  R tmp = R(k::HC6FourPiSq) * R(k::BremC4BRT06) * (q / (T * T * T * T * T * T * T)) *
+          bessk1(x) / x;
+  s_em = tmp * safe_exp(-x);
+  s_abs = tmp * safe_exp(x);
+}
+
+// BremKernelS, kernel_brem_HR98.hpp:33-152
+template <class R>
+inline R brem_s(R x, R y, R es) {
+  x = (x > R(k::BremXmin)) ? x : R(k::BremXmin);
+  y = (y > R(k::BremYmin)) ? y : R(k::BremYmin);
+  es = (es > R(k::BremEtamin)) ? es : R(k::BremEtamin);
+  R snd_num = R(2.) * R(k::SqrtPi) * rpow(x + R(2.) - safe_exp(-y / R(12.)), R(1.5)) *
+              (x * x + R(2.) * x * y + R(5.) * (y * y) / R(3.) + R(1.));
+  R b = R(k::Pi2OneEighth) + x + y;
+  R snd_den = R(k::SqrtPi) + b * b * b * b;
+  R s_nd = snd_num / snd_den;
+  R u = rsqrt(y / (R(2.) * es)) + R(1.e-10);
+  R u2 = u * u;
+  R u_arg = u2 / (R(2.) * rsqrt(R(2.) * u2 + R(4.)));
+  R f_u = (R(1.) - R(5.) * u * ratan(R(2.) / u) / R(6.) + u2 / (R(3.) * (u2 + R(4.))) +
+           ratan(R(1.) / u_arg) * u_arg / R(3.));
+  f_u = (rfabs(f_u) < R(1.e-14)) ? R(1.e-14) : f_u;
+  R s_d = R(3.) * R(k::PiHalfToFiveHalves) * rpow(es, -R(2.5)) * (x * x + R(k::FourPiSquared)) *
+          x * f_u / (R(k::FourPiSquared) * (R(1.) - safe_exp(-x)));
+  R px11 = rpow(x, R(1.1));
+  R xm = x - R(1.2);
+  R f_den = (R(3.) + xm * xm + rpow(x, -R(4.))) * (R(1.) + es * es) * (R(1.) + y * y * y * y);
+  R f_brem = R(1.) + R(1.) / f_den;
+  R g_brem = R(1.) - R(0.0044) * px11 * y / (R(0.8) + R(0.06) * rpow(y, R(1.05))) * rsqrt(es) /
+                         (es + R(0.2));
+  R h_brem = R(0.1) * es / (R(2.39) + R(0.1) * rpow(es, R(1.1)));
+  R c_brem = R(1.1) * px11 * h_brem /
+             (R(2.3) + h_brem * rpow(x, R(0.93)) + R(0.0001) * rpow(x, R(1.2))) * R(30.) /
+             (R(30.) + R(0.005) * rpow(x, R(2.8)));
+  R p_brem = R(0.67) + R(0.18) * rpow(y, R(0.4));
+  return rpow(rpow(s_nd, -p_brem) + rpow(s_d, -p_brem), -R(1.) / p_brem) * f_brem *
+         (R(1.) + c_brem * g_brem);
+}
+
+// BremKernelG, kernel_brem_HR98.hpp:165-236
+template <class R>
+inline R brem_g(R y, R es) {
+  y = (y > R(k::BremYmin)) ? y : R(k::BremYmin);
+  es = (es > R(k::BremEtamin)) ? es : R(k::BremEtamin);
+  R y2 = y * y;
+  R ei = R(1.) / es;
+  R a1d = R(25.) * y2 + R(1.);
+  R alpha1 = (R(0.5) + ei) / (R(1.) + ei) * (R(1.) / a1d) + (R(0.5) + es / R(15.6)) * R(25.) * y2 / a1d;
+  R alpha2 = (R(0.63) + R(0.04) * rpow(es, R(1.45))) / (R(1.) + R(0.02) * rpow(es, R(2.5)));
+  R es15 = rpow(es, R(1.5));
+  R alpha3 = R(1.2) * safe_exp(R(0.6) * es - R(0.4) * es15);
+  R p1 = (R(1.8) + R(0.45) * es) / (R(1.) + R(0.15) * es15);
+  R p2 = R(2.3) - R(0.05) * es / (R(1.) + R(0.025) * es);
+  return (alpha1 + alpha2 * rpow(y, p1)) /
+         (R(1.) + alpha3 * rpow(y, p2) + alpha2 * rpow(y, p1 + R(2.)) / R(13.75));
+}
+
+// BremSingleChannelAbsKernel, kernel_brem_HR98.hpp:239-289
+template <class R>
+inline R brem_channel(R n_nuc, R m_nuc, R omega, R omega_p, R T) {
+  R x = (omega + omega_p) / T;
+  R T10 = T * R(0.1);
+  R es = rpow(R(3.) * R(k::PiSquared) * n_nuc, R(2. / 3.)) * R(k::BremAux1) / (R(2.) * m_nuc * T);
+  R gamma = R(1.63) * rpow(es, R(1.5)) * T10;
+  R y = R(1.94) / T10;
+  R sb = brem_s(x, y, es);
+  R gb = brem_g(y, es);
+  R hg = R(0.5) * gamma * gb;
+  return gamma / (x * x + hg * hg) * sb / T;
+}
+
+// BremAllChannelsAbsKernel + BremKernelsLegCoeff (l = 0),
+// kernel_brem_HR98.hpp:293-393
+template <class R>
+inline void brem_hr98(R omega, R omega_p, const Eos<R>& eos, bool medium, R& s_em, R& s_abs) {
+  R xn = eos.yn, xp = eos.yp, nb = eos.nb;
+  R x_mean = rsqrt(xn * xp);
+  R nn = nb * xn, np = nb * xp, n_mean = nb * x_mean;
+  R s_nn = brem_channel(nn, R(k::MnGrams), omega, omega_p, eos.T);
+  R s_pp = brem_channel(np, R(k::MpGrams), omega, omega_p, eos.T);
+  R s_np = brem_channel(n_mean, R(k::MAvgGrams), omega, omega_p, eos.T);
+  R tot = R(k::BremConst) * (nn * s_nn + np * s_pp + R(28. / 3.) * n_mean * s_np);
+  if (medium) {
+    R c = R(1.) + rcbrt(nb / R(k::SaturationN)) / R(3.);
+    tot = tot / (c * c * c * c * c * c);
+  }
+  R x = (omega + omega_p) / eos.T;
+  s_abs = R(3.) * tot;
+  s_em = s_abs * safe_exp(-x);
+}
+
+// ---------------------------------------------------------------------------
+// Inelastic scattering on e- / e+ (kernel_nes.hpp).
+// G[k][a] are F_k at a = 0: eta-wp, 1: eta-w, 2: eta-|w-wp|, 3: eta.
+// ---------------------------------------------------------------------------
+// MezzacappaIntOut, kernel_nes.hpp:50-78
+template <class R>
+inline R mezz_out(R w, R wp, R x, R y, int sign, R b1, R b2, const R fdw[5], const R fda[3]) {
+  R sg = R(double(sign));
+  return (((b1 + b2) * (sg * fda[2] - fdw[4]) * R(0.2) -
+           b1 * (w + wp) * (fdw[3] + R(2.) * ((w + wp) * fdw[2] + R(3.) * w * wp * fdw[1])) +
+           sg * ((b1 * x - b2 * y) * fda[1] + R(2.) * (b1 * x * x + b2 * y * y) * fda[0])) /
+              (w * w * wp * wp) -
+          R(6.) * b1 * fdw[0]);
+}
+// MezzacappaIntOneEnergy, kernel_nes.hpp:82-107
+template <class R>
+inline R mezz_one(R x, R y, int sign, R b1, R b2, const R f[5]) {
+  return R(double(-sign)) * y *
+         (R(2.) * (b1 + b2) * f[0] + (b1 * (y + R(4.) * x) + R(3.) * b2 * y) * f[1] +
+          (b1 * (R(3.) * y - R(4.) * x) + b2 * y) * f[2] +
+          ((b1 + R(6.) * b2) * y * y * R(0.2) + b1 * x * y + R(2.) * b1 * x * x) * f[3] -
+          ((R(6.) * b1 + b2) * y * y * R(0.2) - R(3.) * b1 * x * y + R(2.) * b1 * x * x) * f[4]) /
+         (x * x);
+}
+// MezzacappaIntTwoEnergies, kernel_nes.hpp:111-132
+template <class R>
+inline R mezz_two(R w, R wp, R x, R y, R b1, R b2, const R f[3]) {
+  return y * (w - wp) *
+         ((b1 + b2) * (R(4.) * f[2] + f[0] * (R(11.) * y * y - R(25.) * x * y + R(20.) * x * x) / R(30.)) +
+          (b1 - b2) * (R(2.) * x - y) * f[1]) /
+         (x * x);
+}
+
+// Per-point scalars shared by every NES/NPS cell
+template <class R>
+struct NepsPoint {
+  R T, eta[2];       // eta for electrons (0) and positrons (1)
+  R Fe[2][6];        // F_k(eta), k = 0..5, per lepton
+  R fd0[2];          // FermiDistr(0, 1, eta)
+};
+
+// NESKernels / NPSKernels, kernel_nes.hpp:139-340, for one lepton (lep = 0
+// electrons, 1 positrons) and the energy pair (omega, omega').  The F values
+// at eta-wp, eta-w and eta-|w-wp| are supplied by the caller (G[k][0..2],
+// k = 1..5) so that the swapped call reuses them.
+template <class R>
+inline void neps_one_lepton(int lep, R w, R wp, const NepsPoint<R>& P, const R G[6][3],
+                            R em[4], R ab[4]) {
+  const R b[3] = {R(k::NEPSBPlus), R(k::NEPSBZero), R(k::NEPSBMinus)};
+  // couplings (b1, b2) per species; NES: (B+,B0),(B0,B+),(B-,B0),(B0,B-),
+  // NPS swaps the two entries (kernel_nes.hpp:167-182, 269-284)
+  static const int cpl[2][4][2] = {{{0, 1}, {1, 0}, {2, 1}, {1, 2}},
+                                   {{1, 0}, {0, 1}, {1, 2}, {2, 1}}};
+  const R T = P.T;
+  const R eta = P.eta[lep];
+  R x = (w > wp) ? w : wp;
+  R y = (w < wp) ? w : wp;
+  R ef = neps_exp_func(wp - w);
+  R efx = neps_exp_func(w - wp);
+  R pre = R(k::NEPSConst) * (T * T);
+  R out[4];
+  if (y > eta * R(1e-3)) {
+    int sign = 2 * (rsignbit(wp - w) ? 1 : 0) - 1;
+    R fdw[5], fda[3];
+    for (int kk = 1; kk <= 5; ++kk) fdw[kk - 1] = G[kk][0] - G[kk][1];
+    for (int kk = 3; kk <= 5; ++kk) fda[kk - 3] = P.Fe[lep][kk] - G[kk][2];
+    for (int s = 0; s < 4; ++s)
+      out[s] = pre * mezz_out(w, wp, x, y, sign, b[cpl[lep][s][0]], b[cpl[lep][s][1]], fdw, fda);
+  } else if (x > eta * R(1e-3)) {
+    int sign = 2 * (rsignbit(wp - w) ? 1 : 0) - 1;
+    R f[5] = {fdi(2, eta - x) - P.Fe[lep][2], fdi(1, eta - x), P.Fe[lep][1],
+              fdi(0, eta - x) + y * fermi_distr(R(0.), R(1.), eta - x) / R(6.),
+              P.Fe[lep][0] - y * P.fd0[lep] / R(6.)};
+    for (int s = 0; s < 4; ++s)
+      out[s] = pre * mezz_one(x, y, sign, b[cpl[lep][s][0]], b[cpl[lep][s][1]], f);
+  } else {
+    R f[3] = {P.fd0[lep], P.Fe[lep][0], P.Fe[lep][1]};
+    for (int s = 0; s < 4; ++s)
+      out[s] = pre * mezz_two(w, wp, x, y, b[cpl[lep][s][0]], b[cpl[lep][s][1]], f);
+  }
+  for (int s = 0; s < 4; ++s) {
+    em[s] = -out[s] * efx;
+    ab[s] = out[s] * ef;
+  }
+}
+
+// F_k at eta-w' (slot 0), eta-w (slot 1) and eta-|w-w'| (slot 2) for both
+// leptons; evaluated once and shared by the (w,w') and (w',w) kernels.
+template <class R>
+inline void neps_fdi_table(R w, R wp, const NepsPoint<R>& P, R G[2][6][3]) {
+  R ad = rfabs(w - wp);
+  R y = (w < wp) ? w : wp;
+  for (int lep = 0; lep < 2; ++lep) {
+    R eta = P.eta[lep];
+    for (int kk = 0; kk < 6; ++kk) G[lep][kk][0] = G[lep][kk][1] = G[lep][kk][2] = R(0.);
+    if (!(y > eta * R(1e-3))) continue;  // only the full branch reads them
+    R b0[6], b1[6], b2[6];
+    fdi_bundle(1, eta - wp, b0);
+    fdi_bundle(1, eta - w, b1);
+    fdi_bundle(3, eta - ad, b2);
+    for (int kk = 1; kk <= 5; ++kk) {
+      G[lep][kk][0] = b0[kk];
+      G[lep][kk][1] = b1[kk];
+      G[lep][kk][2] = kk >= 3 ? b2[kk] : R(0.);
+    }
+  }
+}
+
+// InelasticScattKernels, kernel_nes.hpp:343-359, for (omega, omega') given
+// the shared table G (computed for that same ordered pair) -- when `swap` is
+// set, the kernel for (omega', omega) is produced from the same table.
+template <class R>
+inline void inel_kernels(R omega, R omega_p, const NepsPoint<R>& P, const R G[2][6][3],
+                         bool swap, KOut<R>& out) {
+  R w = omega / P.T, wp = omega_p / P.T;
+  R em_l[2][4], ab_l[2][4];
+  for (int lep = 0; lep < 2; ++lep) {
+    R Gs[6][3];
+    for (int kk = 0; kk < 6; ++kk) {
+      Gs[kk][0] = swap ? G[lep][kk][1] : G[lep][kk][0];
+      Gs[kk][1] = swap ? G[lep][kk][0] : G[lep][kk][1];
+      Gs[kk][2] = G[lep][kk][2];
+    }
+    if (swap) neps_one_lepton(lep, wp, w, P, Gs, em_l[lep], ab_l[lep]);
+    else neps_one_lepton(lep, w, wp, P, Gs, em_l[lep], ab_l[lep]);
+  }
+  for (int s = 0; s < 4; ++s) {
+    out.em[s] = em_l[0][s] + em_l[1][s];
+    out.abs[s] = ab_l[0][s] + ab_l[1][s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Distribution function (distribution.hpp)
+// ---------------------------------------------------------------------------
+// CalculateThickParamsFromM1 + CalculateThinParamsFromM1,
+// distribution.hpp:51-243
+template <class R>
+inline void distr_from_m1(const M1<R>& m1, Distr<R>& d) {
+  static const double n1[6] = {0.19926987701997, 38865.0149220478, 14364.6331099737,
+                               5750.1878570758, 1120.71610972194, 1.60356108438235e-8};
+  static const double n2[7] = {41.3836568203438, 32.5515666786612, 157.774993512235,
+                               66.5726772426253, 14.4883415579211, 0.315360380575709,
+                               0.000660414331285249};
+  static const double n3[7] = {3852.81416018959, 5316.18895799799, 1102.91561586553,
+                               1.54082262710661e-6, 1732.89925128741, 1769.59868329086,
+                               586.406885304906};
+  static const double d1[6] = {1.0, 38840.0743174942, 99.9009680656931, 171.874844843596,
+                               75.7101579899442, 83.0160130941424};
+  static const double d2[6] = {1.8888797407042, 5.35488690539183, 1.94673781342617,
+                               0.483128792035557, 0.0113386564109086, 2.64160073447322e-5};
+  static const double d3[6] = {255.936658313629, 9.42360945627147e-5, 81.2467063138386,
+                               180.100197053091, 89.0343496217014, 143.849128123195};
+  const R y1 = R(0.005), y2 = R(0.7), y3 = R(0.7901234567745267);
+  for (int s = 0; s < 4; ++s) {
+    R n = (m1.n[s] > R(1e-100)) ? m1.n[s] : R(1e-100);
+    R J = (m1.J[s] > R(1e-100)) ? m1.J[s] : R(1e-100);
+    R chi = m1.chi[s];
+    // thick part
+    d.w_t[s] = R(1.5) * (R(1.) - chi);
+    R r = n / J;
+    R y = n * (r * r * r) / R(k::FourPiHc3);
+    R et;
+    if (y < y1) {
+      R num = y * (y * (y * (y * (y * (y + R(n1[0])) + R(n1[1])) + R(n1[2])) + R(n1[3])) + R(n1[4])) + R(n1[5]);
+      R den = y * (y * (y * (y * (y * (y + R(d1[0])) + R(d1[1])) - R(d1[2])) - R(d1[3])) + R(d1[4])) + R(d1[5]);
+      et = rlog(num / den);
+    } else if (y <= y2) {
+      R num = y * (y * (y * (y * (y * (R(n2[0]) * y + R(n2[1])) - R(n2[2])) + R(n2[3])) + R(n2[4])) + R(n2[5])) + R(n2[6]);
+      R den = y * (y * (y * (y * (y * (y + R(d2[0])) - R(d2[1])) + R(d2[2])) + R(d2[3])) + R(d2[4])) + R(d2[5]);
+      et = num / den - R(30.);
+    } else if (y > y2 && y < y3) {
+      R num = y * (y * (y * (y * (y * (R(n3[0]) * y - R(n3[1])) + R(n3[2])) + R(n3[3])) + R(n3[4])) - R(n3[5])) + R(n3[6]);
+      R den = y * (y * (y * (y * (y * (y + R(d3[0])) + R(d3[1])) - R(d3[2])) - R(d3[3])) - R(d3[4])) + R(d3[5]);
+      et = rexp(num / den);
+    } else {
+      et = R(20.);
+    }
+    et = (et < R(20.)) ? et : R(20.);
+    d.eta_t[s] = et;
+    d.temp_t[s] = fdi(2, et) * J / (fdi(3, et) * n);
+    // thin part
+    d.w_f[s] = R(0.5) * (R(3.) * chi - R(1.));
+    R cf = R(0.6);
+    d.c_f[s] = cf;
+    R Tnu = J / (n * (cf + R(3.)));
+    d.temp_f[s] = Tnu;
+    d.beta_f[s] = n / (R(k::FourPiHc3) * gamma_stirling(cf + R(3.)) * rpow(Tnu, cf + R(3.)));
+  }
+}
+
+// NuEquilibriumParams, distribution.hpp:253-290
+template <class R>
+inline void distr_equilibrium(const Eos<R>& eos, Distr<R>& d) {
+  for (int s = 0; s < 4; ++s) {
+    d.w_t[s] = R(1.);
+    d.temp_t[s] = eos.T;
+    d.w_f[s] = R(0.);
+    d.c_f[s] = R(0.6);
+    d.temp_f[s] = eos.T;
+    d.beta_f[s] = R(1.);
+  }
+  d.eta_t[0] = (eos.mu_e - eos.mu_n + eos.mu_p) / eos.T;
+  d.eta_t[1] = -d.eta_t[0];
+  d.eta_t[2] = R(0.);
+  d.eta_t[3] = R(0.);
+}
+
+// ComputeM1DensitiesEq, distribution.hpp:447-473
+template <class R>
+inline void m1_densities_eq(const Eos<R>& eos, const Distr<R>& d, M1<R>& m1) {
+  R npre = R(k::FourPiHc3) * (eos.T * eos.T * eos.T);
+  R jpre = npre * eos.T;
+  for (int s = 0; s < 4; ++s) {
+    m1.n[s] = npre * fdi(2, d.eta_t[s]);
+    m1.J[s] = jpre * fdi(3, d.eta_t[s]);
+  }
+}
+
+// TotalNuF (NuFThick + NuFThin), distribution.hpp:32-40, 189-198, 298-315
+template <class R>
+inline R total_nuf(R omega, const Distr<R>& d, int s) {
+  R f_thick = fermi_distr(omega, d.temp_t[s], d.temp_t[s] * d.eta_t[s]);
+  R f_thin = d.beta_f[s] * rpow(omega, d.c_f[s]) * rexp(-omega / d.temp_f[s]);
+  return d.w_t[s] * f_thick + d.w_f[s] * f_thin;
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre nodes/weights on [x1, x2], integration.hpp:28-87 (host)
+// ---------------------------------------------------------------------------
+inline void gauss_legendre(int n, double x1, double x2, double* pts, double* w) {
+  const double eps = 1.0e-10;
+  double z1, z, pp, p3, p2, p1;
+  int m = (n + 1) / 2;
+  double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int i = 0; i < m; ++i) {
+    z = std::cos(k::Pi * (i + 0.75) / (n + 0.5));
+    do {
+      p1 = 1.0;
+      p2 = 0.0;
+      for (int j = 0; j < n; ++j) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j + 1.0) * z * p2 - j * p3) / (j + 1);
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    } while (std::fabs(z - z1) > eps);
+    pts[i] = xm - xl * z;
+    pts[n - 1 - i] = xm + xl * z;
+    w[i] = 2.0 * xl / ((1.0 - z * z) * (pp * pp));
+    w[n - 1 - i] = w[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gray M1 opacities for one point (ComputeM1OpacitiesGenericFormalism,
+// m1_opacities.hpp:1062-1295) with the sub-integrals of :39-237, :289-478,
+// :577-1057 and the quadrature sums of integration.hpp:608-844.
+// ---------------------------------------------------------------------------
+struct Quad {
+  int n;
+  const double* x;
+  const double* w;
+};
+
+template <class R>
+struct Scratch {  // per-thread 2D matrices, [4][N][N] em/ab, N = 2n
+  std::vector<R> em, ab;
+  int N = 0;
+  void ensure(int NN) {
+    if (NN != N) {
+      N = NN;
+      em.assign(size_t(4) * NN * NN, R(0.));
+      ab.assign(size_t(4) * NN * NN, R(0.));
+    }
+  }
+  R& E(int s, int a, int b) { return em[(size_t(s) * N + a) * N + b]; }
+  R& A(int s, int a, int b) { return ab[(size_t(s) * N + a) * N + b]; }
+  void zero() {
+    std::fill(em.begin(), em.end(), R(0.));
+    std::fill(ab.begin(), ab.end(), R(0.));
+  }
+};
+
+// GaussLegendreIntegrate2DMatrixForM1Coeffs, integration.hpp:608-685
+template <class R>
+inline void integrate_2d_m1(const Quad& q, Scratch<R>& M, R t, R r1[8], R r2[8]) {
+  const int n = q.n;
+  for (int s = 0; s < 4; ++s) {
+    for (int i = 0; i < n; ++i) {
+      R xi = R(q.x[i]), wi = R(q.w[i]);
+      R x2i = xi * xi;
+      R aux1 = t * xi, aux2 = t / (xi * x2i);
+      for (int j = 0; j < n; ++j) {
+        R xj = R(q.x[j]), wj = R(q.w[j]);
+        R wij = wi * wj, x2j = xj * xj, x2ij = x2i * x2j;
+        R aux3 = aux1 / x2j, aux4 = aux2 / x2j;
+        r1[s] += wij * (M.E(s, i, j) + M.E(s, n + i, j) / x2i + M.E(s, i, n + j) / x2j +
+                        M.E(s, n + i, n + j) / x2ij);
+        r1[4 + s] += wij * (M.A(s, i, j) + M.A(s, n + i, j) / x2i + M.A(s, i, n + j) / x2j +
+                            M.A(s, n + i, n + j) / x2ij);
+        r2[s] += wij * (M.E(s, i, j) * aux1 + M.E(s, n + i, j) * aux2 + M.E(s, i, n + j) * aux3 +
+                        M.E(s, n + i, n + j) * aux4);
+        r2[4 + s] += wij * (M.A(s, i, j) * aux1 + M.A(s, n + i, j) * aux2 +
+                            M.A(s, i, n + j) * aux3 + M.A(s, n + i, n + j) * aux4);
+      }
+    }
+  }
+  R t2 = t * t;
+  for (int s = 0; s < 8; ++s) {
+    r1[s] *= t2;
+    r2[s] *= t2;
+  }
+}
+
+// GaussLegendreIntegrate2DMatrixForNEPS, integration.hpp:687-768
+template <class R>
+inline void integrate_2d_neps(const Quad& q, Scratch<R>& M, R t, R r1[8], R r2[8]) {
+  const int n = q.n;
+  const R h = R(0.5), one = R(1.);
+  for (int s = 0; s < 4; ++s) {
+    for (int i = 0; i < n; ++i) {
+      R xi = R(q.x[i]), wi = R(q.w[i]);
+      R x3i = xi * xi * xi;
+      for (int j = 0; j < n; ++j) {
+        R xj = R(q.x[j]), wj = R(q.w[j]);
+        R wij = wi * wj;
+        R a1 = h * t * xi * (one - xj), a2 = h * t * xi * (one + xj);
+        R a3 = h * t * (one - xj) / xi, a4 = h * t * (one + xj) / xi;
+        r1[s] += wij * (xi * (M.E(s, i, j) + M.E(s, i, n + j)) +
+                        (M.E(s, n + i, j) + M.E(s, n + i, n + j)) / x3i);
+        r1[4 + s] += wij * (xi * (M.A(s, i, j) + M.A(s, i, n + j)) +
+                            (M.A(s, n + i, j) + M.A(s, n + i, n + j)) / x3i);
+        r2[s] += wij * (xi * (a1 * M.E(s, i, j) + a2 * M.E(s, i, n + j)) +
+                        (a3 * M.E(s, n + i, j) + a4 * M.E(s, n + i, n + j)) / x3i);
+        r2[4 + s] += wij * (xi * (a1 * M.A(s, i, j) + a2 * M.A(s, i, n + j)) +
+                            (a3 * M.A(s, n + i, j) + a4 * M.A(s, n + i, n + j)) / x3i);
+      }
+    }
+  }
+  R f = h * (t * t);
+  for (int s = 0; s < 8; ++s) {
+    r1[s] *= f;
+    r2[s] *= f;
+  }
+}
+
+// Pair + bremsstrahlung matrices with distribution weights
+// (ComputeDoubleIntegrand, m1_opacities.hpp:774-862, with
+// AddPairKernelsToIntegrand :289-340, AddBremKernelsToIntegrand :343-478,
+// WeightNuNuBarReactionsWithDistr :577-660, AddCommonWeightsToIntegrand
+// :662-758).  Node Fermi integrals and node distribution values are
+// tabulated once.
+template <class R>
+inline void double_integrand(const Quad& q, R t, const Eos<R>& eos, const Distr<R>& d,
+                             const Flags& fl, const Params& pr, int stim, Scratch<R>& M) {
+  const int n = q.n, N = 2 * n;
+  M.ensure(N);
+  M.zero();
+  std::vector<R> nu(N);
+  for (int i = 0; i < n; ++i) {
+    nu[i] = t * R(q.x[i]);
+    nu[n + i] = t / R(q.x[i]);
+  }
+  const R T = eos.T;
+  if (fl.use_pair == 1) {
+    const R eta = eos.mu_e / T;
+    // node table: F_k(eta -+ y_a), k = 1..5; per-point F_k(eta), k = 3..5
+    std::vector<R> yv(N), Fm(size_t(N) * 6), Fp(size_t(N) * 6);
+    for (int a = 0; a < N; ++a) {
+      yv[a] = nu[a] / T;
+      fdi_bundle(1, eta - yv[a], &Fm[size_t(a) * 6]);
+      fdi_bundle(1, eta + yv[a], &Fp[size_t(a) * 6]);
+    }
+    R Fe[6];
+    fdi_bundle(3, eta, Fe);
+    for (int a = 0; a < N; ++a) {
+      for (int b = a; b < N; ++b) {
+        R y = yv[a], z = yv[b];
+        R F[6][7];
+        for (int kk = 1; kk <= 5; ++kk) {
+          F[kk][A_EMY] = Fm[size_t(a) * 6 + kk];
+          F[kk][A_EPY] = Fp[size_t(a) * 6 + kk];
+          F[kk][A_EMZ] = Fm[size_t(b) * 6 + kk];
+          F[kk][A_EPZ] = Fp[size_t(b) * 6 + kk];
+        }
+        if (!(eta - y - z > R(200.))) {
+          R bm[6], bp[6];
+          fdi_bundle(3, eta - y - z, bm);
+          fdi_bundle(3, eta + y + z, bp);
+          for (int kk = 3; kk <= 5; ++kk) {
+            F[kk][A_E] = Fe[kk];
+            F[kk][A_EMYZ] = bm[kk];
+            F[kk][A_EPYZ] = bp[kk];
+          }
+        }
+        KOut<R> fwd;
+        pair_kernel_fwd(nu[a], nu[b], T, eta, y, z, F, fwd);
+        for (int s = 0; s < 4; ++s) {
+          M.E(s, a, b) += fwd.em[s];
+          M.A(s, a, b) += fwd.abs[s];
+          if (b != a) {  // inverse kernel: species swapped (kernel_pair.hpp:207-215)
+            M.E(s, b, a) += fwd.em[partner(s)];
+            M.A(s, b, a) += fwd.abs[partner(s)];
+          }
+        }
+      }
+    }
+  }
+  if (fl.use_brem == 1 && pr.brem_impl != BREM_GP19) {
+    for (int a = 0; a < N; ++a) {
+      for (int b = a; b < N; ++b) {
+        R se, sa;
+        if (pr.brem_impl == BREM_BRT06) brem_brt06(nu[a], nu[b], eos, se, sa);
+        else brem_hr98(nu[a], nu[b], eos, pr.use_NN_medium_corr != 0, se, sa);
+        for (int s = 0; s < 4; ++s) {
+          M.E(s, a, b) += se;
+          M.A(s, a, b) += sa;
+          if (b != a) {
+            M.E(s, b, a) += se;
+            M.A(s, b, a) += sa;
+          }
+        }
+      }
+    }
+  }
+  // node distribution values
+  std::vector<R> g(size_t(N) * 4);
+  for (int a = 0; a < N; ++a)
+    for (int s = 0; s < 4; ++s) g[size_t(a) * 4 + s] = total_nuf(nu[a], d, s);
+  auto G = [&](int a, int s) { return g[size_t(a) * 4 + s]; };
+  auto BF = [&](int a, int s) { return pr.neglect_blocking ? R(1.) : R(1.) - G(a, s); };
+  if (fl.use_pair == 1 || fl.use_brem == 1) {
+    for (int a = 0; a < N; ++a) {
+      for (int b = a; b < N; ++b) {
+        for (int s = 0; s < 4; ++s) {
+          int ps = partner(s);
+          M.E(s, a, b) *= BF(b, ps);
+          M.A(s, a, b) *= G(b, ps);
+          if (b != a) {
+            M.E(s, b, a) *= BF(a, ps);
+            M.A(s, b, a) *= G(a, ps);
+          }
+        }
+      }
+    }
+  }
+  for (int a = 0; a < N; ++a) {
+    R nsq = nu[a] * nu[a];
+    for (int b = a; b < N; ++b) {
+      R n4 = (b == a) ? nsq * nsq : nsq * (nu[b] * nu[b]);
+      for (int s = 0; s < 4; ++s) {
+        if (stim == 1) {
+          M.A(s, a, b) = n4 * G(a, s) * (M.E(s, a, b) + M.A(s, a, b));
+          if (b != a) M.A(s, b, a) = n4 * G(b, s) * (M.E(s, b, a) + M.A(s, b, a));
+          M.E(s, a, b) *= n4;
+          if (b != a) M.E(s, b, a) *= n4;
+        } else {
+          M.A(s, a, b) *= n4 * G(a, s);
+          if (b != a) M.A(s, b, a) *= n4 * G(b, s);
+          M.E(s, a, b) *= n4 * (R(1.) - G(a, s));
+          if (b != a) M.E(s, b, a) *= n4 * (R(1.) - G(b, s));
+        }
+      }
+    }
+  }
+}
+
+// ComputeNEPSIntegrand, m1_opacities.hpp:864-1057
+template <class R>
+inline void neps_integrand(const Quad& q, R t, const Eos<R>& eos, const Distr<R>& d,
+                           const Params& pr, int stim, Scratch<R>& M) {
+  const int n = q.n;
+  M.ensure(2 * n);
+  NepsPoint<R> P;
+  P.T = eos.T;
+  P.eta[0] = eos.mu_e / eos.T;
+  P.eta[1] = -eos.mu_e / eos.T;
+  for (int lep = 0; lep < 2; ++lep) {
+    R b[6];
+    fdi_bundle(0, P.eta[lep], b);
+    for (int kk = 0; kk < 6; ++kk) P.Fe[lep][kk] = b[kk];
+    P.fd0[lep] = fermi_distr(R(0.), R(1.), P.eta[lep]);
+  }
+  const R h = R(0.5), one = R(1.);
+  for (int i = 0; i < n; ++i) {
+    R xi = R(q.x[i]);
+    for (int j = 0; j < n; ++j) {
+      R xj = R(q.x[j]);
+      for (int half = 0; half < 2; ++half) {
+        R nu, nub;
+        if (half == 0) {
+          nu = h * t * xi * (one - xj);
+          nub = h * t * xi * (one + xj);
+        } else {
+          nu = h * t * (one - xj) / xi;
+          nub = h * t * (one + xj) / xi;
+        }
+        R n4 = (nu * nu) * (nub * nub);
+        R gn[4], gb[4], bfn[4], bfb[4];
+        for (int s = 0; s < 4; ++s) {
+          gn[s] = total_nuf(nu, d, s);
+          gb[s] = total_nuf(nub, d, s);
+          bfn[s] = pr.neglect_blocking ? R(1.) : R(1.) - gn[s];
+          bfb[s] = pr.neglect_blocking ? R(1.) : R(1.) - gb[s];
+        }
+        R G[2][6][3];
+        R w = nu / eos.T, wp = nub / eos.T;
+        neps_fdi_table(w, wp, P, G);
+        KOut<R> k1, k2;
+        inel_kernels(nu, nub, P, G, false, k1);
+        inel_kernels(nu, nub, P, G, true, k2);
+        int r = half == 0 ? i : n + i;
+        for (int s = 0; s < 4; ++s) {
+          R te1 = k1.em[s] * gb[s], ta1 = k1.abs[s] * bfb[s];
+          R te2 = k2.em[s] * gn[s], ta2 = k2.abs[s] * bfn[s];
+          if (stim == 1) {
+            M.A(s, r, j) = n4 * gn[s] * (te1 + ta1);
+            M.E(s, r, j) = n4 * te1;
+            M.A(s, r, n + j) = n4 * gb[s] * (te2 + ta2);
+            M.E(s, r, n + j) = n4 * te2;
+          } else {
+            M.A(s, r, j) = n4 * gn[s] * ta1;
+            M.E(s, r, j) = n4 * (one - gn[s]) * te1;
+            M.A(s, r, n + j) = n4 * gb[s] * ta2;
+            M.E(s, r, n + j) = n4 * (one - gb[s]) * te2;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <class R>
+struct PointResult {
+  R out[20];
+};
+
+// ComputeM1OpacitiesGenericFormalism, m1_opacities.hpp:1062-1295
+template <class R>
+inline void m1_opacities_point(const Quad& q1, const Quad& q2, const Eos<R>& eos,
+                               const M1<R>& m1, const Distr<R>& d, const Flags& fl,
+                               const Params& pr, int stim, Scratch<R>& M, R out[20]) {
+  const R c = R(k::Clight);
+  const R temp = eos.T;
+  const R eta_e = eos.mu_e / temp;
+  const R tm = R(0.5 * 4.364);
+  const R s_pair = tm * temp;
+  const R s_nux = R(1.5) * temp;
+  const R s_neps = tm * temp;
+  R s_beta[4] = {R(0.), R(0.), R(0.), R(0.)};
+  if (eta_e > R(-30.) && eta_e < R(30.)) {
+    s_beta[NUE] = temp * fdi(5, eta_e) / fdi(4, eta_e);
+    s_beta[ANUE] = temp * fdi(5, -eta_e) / fdi(4, -eta_e);
+  } else if (eta_e > R(30.)) {
+    s_beta[NUE] = temp * eta_e * R(0.8333333333333333);
+    s_beta[ANUE] = temp * R(5.);
+  } else {
+    s_beta[NUE] = temp * R(5.);
+    s_beta[ANUE] = temp * eta_e * R(0.8333333333333333);
+  }
+  R s_iso[4];
+  for (int s = 0; s < 4; ++s) s_iso[s] = (m1.n[s] > R(k::ThresholdN)) ? (m1.J[s] / m1.n[s]) : s_nux;
+
+  // isoenergetic scattering (Scattering1DIntegrand :39-91 +
+  // GaussLegendreIntegrate1DMatrix, integration.hpp:770-802)
+  R iso[4] = {R(0.), R(0.), R(0.), R(0.)};
+  if (fl.use_iso == 1) {
+    const int n = q1.n;
+    std::vector<R> o(size_t(4) * 2 * n);
+    const R four_pi = R(4. * k::Pi);
+    for (int i = 0; i < n; ++i) {
+      for (int s = 0; s < 4; ++s) {
+        for (int hlf = 0; hlf < 2; ++hlf) {
+          R nu = hlf == 0 ? s_iso[s] * R(q1.x[i]) : s_iso[s] / R(q1.x[i]);
+          R g = total_nuf(nu, d, s);
+          R isc = iso_scatt_total(nu, pr, eos);
+          R aux = four_pi * (nu * nu) * (nu * nu) * nu * isc;
+          o[size_t(s) * 2 * n + hlf * n + i] = g * aux;
+        }
+      }
+    }
+    for (int s = 0; s < 4; ++s) {
+      for (int i = 0; i < n; ++i) {
+        R xi = R(q1.x[i]);
+        R x2 = xi * xi;
+        iso[s] += R(q1.w[i]) * (o[size_t(s) * 2 * n + i] + o[size_t(s) * 2 * n + n + i] / x2);
+      }
+      iso[s] *= s_iso[s];
+    }
+  }
+
+  // beta processes (Beta1DIntegrand :93-237 +
+  // GaussLegendreIntegrate1DMatrixOnlyNumber, integration.hpp:805-844)
+  R bne[2] = {R(0.), R(0.)}, bje[2] = {R(0.), R(0.)}, bna[2] = {R(0.), R(0.)}, bja[2] = {R(0.), R(0.)};
+  if (fl.use_abs_em == 1) {
+    const int n = q1.n;
+    std::vector<R> oe(size_t(2) * 2 * n), oa(size_t(2) * 2 * n);
+    for (int i = 0; i < n; ++i) {
+      for (int hlf = 0; hlf < 2; ++hlf) {
+        for (int s = 0; s < 2; ++s) {
+          R nu = hlf == 0 ? s_beta[s] * R(q1.x[i]) : s_beta[s] / R(q1.x[i]);
+          R nsq = nu * nu;
+          R g = total_nuf(nu, d, s);
+          R em[2], ab[2];
+          abs_opacity(nu, pr, eos, em, ab);
+          if (stim == 1) ab[s] = ab[s] + em[s];  // StimAbsOpacity, formulas_beta_iso.hpp:330-341
+          oe[size_t(s) * 2 * n + hlf * n + i] = nsq * em[s];
+          oa[size_t(s) * 2 * n + hlf * n + i] = nsq * g * ab[s];
+        }
+      }
+    }
+    for (int s = 0; s < 2; ++s) {
+      for (int i = 0; i < n; ++i) {
+        R xi = R(q1.x[i]), wi = R(q1.w[i]);
+        R x2 = xi * xi;
+        const R* me = &oe[size_t(s) * 2 * n];
+        const R* ma = &oa[size_t(s) * 2 * n];
+        bne[s] += wi * (me[i] + me[n + i] / x2);
+        bje[s] += wi * (me[i] * s_beta[s] * xi + me[n + i] * (s_beta[s] / xi) / x2);
+        bna[s] += wi * (ma[i] + ma[n + i] / x2);
+        bja[s] += wi * (ma[i] * s_beta[s] * xi + ma[n + i] * (s_beta[s] / xi) / x2);
+      }
+      bne[s] *= s_beta[s];
+      bje[s] *= s_beta[s];
+      bna[s] *= s_beta[s];
+      bja[s] *= s_beta[s];
+    }
+  }
+
+  R n2d[8] = {}, e2d[8] = {};
+  for (int s = 0; s < 8; ++s) n2d[s] = e2d[s] = R(0.);
+  if (fl.use_pair == 1 || fl.use_brem == 1) {
+    double_integrand(q2, s_pair, eos, d, fl, pr, stim, M);
+    integrate_2d_m1(q2, M, s_pair, n2d, e2d);
+  }
+  R nnp[8], enp[8];
+  for (int s = 0; s < 8; ++s) nnp[s] = enp[s] = R(0.);
+  if (fl.use_inelastic_scatt == 1) {
+    neps_integrand(q2, R(4.) * s_neps, eos, d, pr, stim, M);
+    integrate_2d_neps(q2, M, R(4.) * s_neps, nnp, enp);
+  }
+
+  const R F = R(k::FourPiHc3), F2 = R(k::FourPiHc3Sqr);
+  const R* n = m1.n;
+  const R* J = m1.J;
+  for (int s = 0; s < 20; ++s) out[s] = R(0.);
+  // electron neutrinos and antineutrinos (m1_opacities.hpp:1187-1244)
+  for (int s = 0; s < 2; ++s) {
+    out[O_ETA0 + s] = F * (F * (n2d[s] + nnp[s]) + bne[s]);
+    out[O_ETA + s] = F * (F * (e2d[s] + enp[s]) + bje[s]);
+    if (n[s] > R(k::ThresholdN))
+      out[O_KAPPA0A + s] = F / (c * n[s]) * (F * (n2d[4 + s] + nnp[4 + s]) + bna[s]);
+    if (J[s] > R(k::ThresholdJ)) {
+      R ka = F / (c * J[s]) * (F * (e2d[4 + s] + enp[4 + s]) + bja[s]);
+      // the extra n == 0 guard exists for nue only (:1206-1208)
+      out[O_KAPPAA + s] = (s == NUE && rval(n[s]) == 0.) ? R(0.) : ka;
+      out[O_KAPPAS + s] = F / (c * J[s]) * iso[s];
+    }
+  }
+  // heavy neutrinos and antineutrinos (m1_opacities.hpp:1246-1292)
+  for (int s = 2; s < 4; ++s) {
+    out[O_ETA0 + s] = F2 * (n2d[s] + nnp[s]);
+    out[O_ETA + s] = F2 * (e2d[s] + enp[s]);
+    if (n[s] > R(k::ThresholdN)) {
+      R k0 = F2 / (c * n[s]) * (n2d[4 + s] + nnp[4 + s]);
+      // the extra n == 0 guard exists for anux only (:1277-1279)
+      out[O_KAPPA0A + s] = (s == ANUX && rval(n[s]) == 0.) ? R(0.) : k0;
+    }
+    if (J[s] > R(k::ThresholdJ)) {
+      out[O_KAPPAA + s] = F2 / (c * J[s]) * (e2d[4 + s] + enp[4 + s]);
+      out[O_KAPPAS + s] = F / (c * J[s]) * iso[s];
+    }
+  }
+}
+
+}  // namespace orc
+
+#endif  // BNS_NURATES_AMD_ORACLE_M1_ORACLE_HPP_
