@@ -1,0 +1,178 @@
+"""Throughput of the MI355X gray M1 opacity path on the config-1 workload.
+
+One step = one pass of ComputeM1Opacities (all reactions, BRT06
+bremsstrahlung, Gauss-Legendre n = 16 -> 32 energy nodes) over the 128^3 =
+2,097,152 seeded points of the workload, inputs already resident in HBM.
+Prints one JSON line (the driver's contract):
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Beside the points/s it reports the roofline of the fused kernel (algorithmic
+FP64 FLOPs per launch / kernel time against the FP64 vector peak), the HR98
+variant, and the CPU baseline: the CPU oracle timed on the host cores on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "M1-opacity points/s, FP64 plain vs mixed/surrogate at stated rel err; p99 latency"
+
+# Algorithmic work per point of the gray path at n = 16, all reactions, from
+# the FLOP-counting build of the CPU restatement (oracle/flop_count.cpp) on the
+# config-1 distributions (+, -, *, /, sqrt = 1 FLOP each; exp/log/pow/atan/cbrt
+# tallied apart).  tests/test_bench_constants.py re-derives them.
+FLOPS_PER_POINT = {"brt06": 1114336.0, "hr98": 1361608.0}
+TRANSCENDENTALS_PER_POINT = {"brt06": {"exp": 19309.3, "pow": 5528.0, "log": 197.7},
+                             "hr98": {"exp": 23201.3, "pow": 38264.0, "log": 1.7,
+                                      "atan": 3168.0, "cbrt": 528.0}}
+PEAK_FP64_VECTOR_TFLOPS = 78.6  # MI355X datasheet, FP64 vector (FMA = 2)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pts, nquad, params, target_s=10.0):
+    """The CPU oracle (FP64, OpenMP over points) on a bounded prefix of the
+    workload, all threads the host grants (OMP_NUM_THREADS or nproc)."""
+    import oracle as O
+    from bns_nurates_amd import workloads as W
+    if not O.available("orc"):
+        O.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    probe = np.ascontiguousarray(pts[:, : 64 * threads])
+    t0 = time.perf_counter()
+    O.m1_opacities(probe, nquad, W.FLAGS_ALL, params, nthreads=threads)
+    rate = probe.shape[1] / (time.perf_counter() - t0)
+    n = int(min(max(rate * target_s, 1024), pts.shape[1]))
+    sample = np.ascontiguousarray(pts[:, :n])
+    t0 = time.perf_counter()
+    O.m1_opacities(sample, nquad, W.FLAGS_ALL, params, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} points of the workload, n={nquad}, FP64, {dt:.1f} s wall "
+                      f"on {threads} threads (oracle/m1_oracle.hpp, bit-exact with the "
+                      f"reference at n<=10)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--npts", type=int, default=128 ** 3)
+    ap.add_argument("--nquad", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-hr98", action="store_true", help="skip the HR98 pass")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="before timing, run one device-to-device copy of this many MiB "
+                         "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(points shard with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bns_nurates_amd import api
+    from bns_nurates_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    log(f"generating {args.npts} points (config-1 distributions, seed 1)")
+    pts = W.make_points(args.npts, seed=1)
+    dpts = torch.from_numpy(pts).cuda()
+    out = torch.empty((W.O_NFIELDS, args.npts), dtype=torch.float64, device="cuda")
+    ctx = api.Context(0)
+    ctx.set_quadrature(args.nquad)
+
+    if args.copy_mib > 0:
+        nel = args.copy_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
+        del src, dst
+
+    def run_pass(params):
+        ctx.compute_m1_opacities(dpts, flags=W.FLAGS_ALL, params=params, out=out)
+
+    res = {}
+    for brem in (["brt06"] if args.no_hr98 else ["brt06", "hr98"]):
+        params = W.params_default_all(W.BREM_BRT06 if brem == "brt06" else W.BREM_HR98)
+        steps = args.steps if brem == "brt06" else max(1, args.steps // 2)
+        for _ in range(args.warmup):
+            run_pass(params)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        log(f"{brem}: timing {steps} steps")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record()
+            run_pass(params)
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        res[brem] = dict(wall=wall, steps=steps, kernel_ms=kern_ms,
+                         pps=args.npts * steps / wall)
+        log(f"{brem}: {res[brem]['pps']:.4g} points/s, kernel {kern_ms:.2f} ms/launch")
+    assert np.isfinite(out.cpu().numpy()).all()
+
+    b = res["brt06"]
+    flops = FLOPS_PER_POINT["brt06"] * args.npts if args.nquad == 16 else None
+    achieved = flops / (b["kernel_ms"] * 1e-3) / 1e12 if flops else None
+    line = {
+        "metric": METRIC,
+        "value": b["pps"],
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": b["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * b["wall"] / b["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "cfg1_grid128_n16_brt06", "npts": args.npts,
+                   "nquad": args.nquad, "energy_nodes": 2 * args.nquad,
+                   "reactions": "beta+pair+brem+nes+iso", "brem": "BRT06",
+                   "opacity_params": "default_all", "distribution": "reconstructed from M1"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VECTOR_TFLOPS,
+                     "unit": "TFLOP/s",
+                     "frac": (achieved / PEAK_FP64_VECTOR_TFLOPS) if achieved else None,
+                     "traffic": None,
+                     "kernel": "k_gray_m1_f64", "kernel_ms": b["kernel_ms"],
+                     "flops_per_point": FLOPS_PER_POINT["brt06"],
+                     "transcendentals_per_point": TRANSCENDENTALS_PER_POINT["brt06"]},
+    }
+    if "hr98" in res:
+        h = res["hr98"]
+        line["hr98"] = {"value": h["pps"], "unit": "points/s", "kernel_ms": h["kernel_ms"],
+                        "achieved_tflops": FLOPS_PER_POINT["hr98"] * args.npts /
+                        (h["kernel_ms"] * 1e-3) / 1e12 if args.nquad == 16 else None}
+    if not args.no_cpu:
+        log("CPU baseline (oracle) on a bounded sample")
+        line["cpu_baseline"] = cpu_baseline(pts, args.nquad, W.params_default_all(W.BREM_BRT06))
+        line["gpu_over_cpu"] = b["pps"] / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

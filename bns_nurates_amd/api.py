@@ -1,0 +1,319 @@
+"""Python host API over the C ABI of libbnsamd.so (include/bns_nurates_amd.h).
+
+Mirrors the reference's host interface for the gray M1 path:
+
+* ``gauss_legendre(n)``             -- ``GaussLegendre`` (integration.hpp:28-87)
+* ``Context.compute_m1_opacities``  -- batched ``ComputeM1Opacities`` /
+  ``ComputeM1OpacitiesNotStimulated`` (m1_opacities.hpp:1297-1313)
+* ``Context.distr_from_m1``         -- ``CalculateDistrParamsFromM1``
+  (distribution.hpp:323-333)
+* ``Context.equilibrium``           -- ``NuEquilibriumParams`` +
+  ``ComputeM1DensitiesEq`` (distribution.hpp:253-290, 447-473)
+
+Point data travel as SoA blocks (21 / 28 / 20 fields of npts float64, the
+field order of the header).  Device blocks are torch tensors on the HIP
+device (torch is used for device memory and streams only); host blocks are
+numpy arrays.  The HIP library is required: there is no CPU fallback, and a
+missing or unloadable library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbnsamd.so")
+
+PT_NFIELDS, DI_NFIELDS, OUT_NFIELDS = 21, 28, 20
+FP64, MIXED = 0, 1
+BREM_HR98, BREM_BRT06, BREM_GP19 = 0, 1, 2
+
+_STATUS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "no HIP device",
+           4: "no quadrature set", 5: "out of device memory"}
+
+
+class BnsAmdError(RuntimeError):
+    pass
+
+
+class _Flags(C.Structure):
+    _fields_ = [(n, C.c_int) for n in
+                ("use_abs_em", "use_pair", "use_brem", "use_inelastic_scatt", "use_iso")]
+
+
+class _Params(C.Structure):
+    _fields_ = [(n, C.c_int) for n in
+                ("use_dU", "use_dm_eff", "use_WM_ab", "use_WM_sc", "use_decay",
+                 "brem_implementation", "use_NN_medium_corr", "neglect_blocking")]
+
+
+@dataclass
+class OpacityFlags:
+    """``OpacityFlags`` (bns_nurates.hpp:418-438)."""
+    use_abs_em: int = 1
+    use_pair: int = 1
+    use_brem: int = 1
+    use_inelastic_scatt: int = 1
+    use_iso: int = 1
+
+    @classmethod
+    def from_array(cls, a):
+        return cls(*[int(v) for v in a])
+
+    def c(self):
+        return _Flags(self.use_abs_em, self.use_pair, self.use_brem, self.use_inelastic_scatt,
+                      self.use_iso)
+
+
+@dataclass
+class OpacityParams:
+    """``OpacityParams`` (bns_nurates.hpp:338-370); defaults are
+    ``opacity_params_default_all`` with the bremsstrahlung model to choose."""
+    use_dU: int = 1
+    use_dm_eff: int = 1
+    use_WM_ab: int = 1
+    use_WM_sc: int = 1
+    use_decay: int = 1
+    brem_implementation: int = BREM_HR98
+    use_NN_medium_corr: int = 1
+    neglect_blocking: int = 1
+
+    @classmethod
+    def from_array(cls, a):
+        return cls(*[int(v) for v in a])
+
+    def c(self):
+        return _Params(self.use_dU, self.use_dm_eff, self.use_WM_ab, self.use_WM_sc,
+                       self.use_decay, self.brem_implementation, self.use_NN_medium_corr,
+                       self.neglect_blocking)
+
+
+def as_flags(flags) -> OpacityFlags:
+    """OpacityFlags from None (all reactions on), a sequence of 5 ints or an
+    OpacityFlags."""
+    if flags is None:
+        return OpacityFlags()
+    if isinstance(flags, OpacityFlags):
+        return flags
+    a = [int(v) for v in np.asarray(flags).ravel()]
+    if len(a) != 5:
+        raise BnsAmdError(f"flags: expected 5 values, got {len(a)}")
+    return OpacityFlags(*a)
+
+
+def as_params(params) -> OpacityParams:
+    """OpacityParams from None (default_all, HR98), a sequence of 8 ints or
+    an OpacityParams."""
+    if params is None:
+        return OpacityParams()
+    if isinstance(params, OpacityParams):
+        return params
+    a = [int(v) for v in np.asarray(params).ravel()]
+    if len(a) != 8:
+        raise BnsAmdError(f"params: expected 8 values, got {len(a)}")
+    return OpacityParams(*a)
+
+
+_lib = None
+
+
+def load_library():
+    """Load libbnsamd.so (build it first: ``make -C bns_nurates_amd/csrc``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BnsAmdError(f"{LIB_PATH} is not built: run make -C bns_nurates_amd/csrc")
+    lib = C.CDLL(LIB_PATH)
+    vp, i64, dp = C.c_void_p, C.c_int64, C.POINTER(C.c_double)
+    sig = {
+        "bnsamd_version": ([], C.c_int),
+        "bnsamd_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+        "bnsamd_destroy": ([vp], None),
+        "bnsamd_last_error": ([vp], C.c_char_p),
+        "bnsamd_set_stream": ([vp, vp], C.c_int),
+        "bnsamd_get_stream": ([vp], vp),
+        "bnsamd_synchronize": ([vp], C.c_int),
+        "bnsamd_gauss_legendre": ([C.c_int, C.c_double, C.c_double, dp, dp], C.c_int),
+        "bnsamd_set_quadrature": ([vp, C.c_int, dp, dp, C.c_int, dp, dp], C.c_int),
+        "bnsamd_m1_opacities": ([vp, i64, i64, vp, vp, C.POINTER(_Flags), C.POINTER(_Params),
+                                 C.c_int, C.c_int, vp], C.c_int),
+        "bnsamd_m1_opacities_host": ([vp, i64, vp, vp, C.POINTER(_Flags), C.POINTER(_Params),
+                                      C.c_int, C.c_int, vp], C.c_int),
+        "bnsamd_distr_from_m1": ([vp, i64, i64, vp, vp], C.c_int),
+        "bnsamd_equilibrium": ([vp, i64, i64, vp, vp], C.c_int),
+        "bnsamd_fdi": ([vp, C.c_int, i64, vp, vp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def gauss_legendre(n: int, x1: float = 0.0, x2: float = 1.0):
+    """Nodes and weights of ``GaussLegendre`` on [x1, x2] (host)."""
+    lib = load_library()
+    x = np.zeros(n)
+    w = np.zeros(n)
+    rc = lib.bnsamd_gauss_legendre(int(n), float(x1), float(x2), _dptr(x), _dptr(w))
+    if rc:
+        raise BnsAmdError(f"bnsamd_gauss_legendre: {_STATUS.get(rc, rc)}")
+    return x, w
+
+
+def _tensor_ptr(t, nfields: int, npts: int, name: str):
+    """Device pointer and leading dimension of a [nfields, >= npts] float64
+    device tensor with unit stride along points."""
+    if t.dtype.__str__() != "torch.float64":
+        raise BnsAmdError(f"{name}: float64 tensor required, got {t.dtype}")
+    if not t.is_cuda:
+        raise BnsAmdError(f"{name}: device tensor required")
+    if t.dim() != 2 or t.shape[0] != nfields or t.shape[1] < npts or t.stride(1) != 1:
+        raise BnsAmdError(f"{name}: expected shape [{nfields}, >= {npts}] with unit point stride")
+    return t.data_ptr(), t.stride(0)
+
+
+class Context:
+    """A device context: stream, resident quadrature tables, staging buffers."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.bnsamd_create(int(device), C.byref(h))
+        if rc:
+            raise BnsAmdError(f"bnsamd_create(device={device}): {_STATUS.get(rc, rc)}")
+        self.h = h
+        self.device = device
+        self._quad = None
+        self._stream_explicit = False
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bnsamd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.bnsamd_last_error(self.h).decode(errors="replace")
+            raise BnsAmdError(f"{what}: {_STATUS.get(rc, rc)}: {msg}")
+
+    def set_stream(self, stream_handle):
+        """Launch on a HIP stream (an int handle such as
+        torch.cuda.current_stream().cuda_stream; 0 is HIP's default stream).
+        None returns to following torch's current stream at every call."""
+        if stream_handle is not None:
+            self._check(self.lib.bnsamd_set_stream(self.h, C.c_void_p(stream_handle)),
+                        "bnsamd_set_stream")
+        self._stream_explicit = stream_handle is not None
+
+    def _follow_torch_stream(self):
+        """Device tensors are produced on torch's current stream: unless a
+        stream was set explicitly, launch there so that no copy is raced."""
+        if not self._stream_explicit:
+            import torch
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            self._check(self.lib.bnsamd_set_stream(self.h, C.c_void_p(s)), "bnsamd_set_stream")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.bnsamd_get_stream(self.h) or 0
+
+    def synchronize(self):
+        self._check(self.lib.bnsamd_synchronize(self.h), "bnsamd_synchronize")
+
+    def set_quadrature(self, n2: int, n1: int | None = None, quad_2d=None, quad_1d=None):
+        """Select the quadratures: orders (with the library's GaussLegendre on
+        [0, 1]) or explicit (x, w) pairs."""
+        n1 = n2 if n1 is None else n1
+        x2, w2 = quad_2d if quad_2d is not None else gauss_legendre(n2)
+        x1, w1 = quad_1d if quad_1d is not None else gauss_legendre(n1)
+        x1, w1, x2, w2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (x1, w1, x2, w2))
+        self._check(self.lib.bnsamd_set_quadrature(self.h, len(x1), _dptr(x1), _dptr(w1),
+                                                   len(x2), _dptr(x2), _dptr(w2)),
+                    "bnsamd_set_quadrature")
+        self._quad = (len(x1), len(x2))
+
+    def compute_m1_opacities(self, pts, flags=None, params=None, stim_abs: int = 1,
+                             precision: int = FP64, distr=None, out=None, npts=None):
+        """Gray M1 opacities [20, npts] of the point block ``pts`` [21, npts].
+
+        Device tensors run asynchronously on the context stream and write
+        ``out`` (allocated when None); numpy arrays go through the
+        synchronous host entry point and return a numpy array."""
+        fl, pr = as_flags(flags).c(), as_params(params).c()
+        if isinstance(pts, np.ndarray):
+            pts = np.ascontiguousarray(pts, dtype=np.float64)
+            n = pts.shape[1] if npts is None else npts
+            res = np.empty((OUT_NFIELDS, n))
+            dd = None
+            if distr is not None:
+                distr = np.ascontiguousarray(distr, dtype=np.float64)
+                dd = distr.ctypes.data
+            self._check(self.lib.bnsamd_m1_opacities_host(
+                self.h, n, pts.ctypes.data, dd, C.byref(fl), C.byref(pr), int(stim_abs),
+                int(precision), res.ctypes.data), "bnsamd_m1_opacities_host")
+            return res
+        import torch
+        self._follow_torch_stream()
+        n = pts.shape[1] if npts is None else npts
+        pp, ld = _tensor_ptr(pts, PT_NFIELDS, n, "pts")
+        dp = None
+        if distr is not None:
+            dp, ldd = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+            if ldd != ld:
+                raise BnsAmdError("distr must have the leading dimension of pts")
+        if out is None:
+            out = torch.empty((OUT_NFIELDS, ld), dtype=torch.float64, device=pts.device)
+        op, ldo = _tensor_ptr(out, OUT_NFIELDS, n, "out")
+        if ldo != ld:
+            raise BnsAmdError("out must have the leading dimension of pts")
+        self._check(self.lib.bnsamd_m1_opacities(self.h, n, ld, pp, dp, C.byref(fl), C.byref(pr),
+                                                 int(stim_abs), int(precision), op),
+                    "bnsamd_m1_opacities")
+        return out
+
+    def distr_from_m1(self, pts, distr=None):
+        import torch
+        self._follow_torch_stream()
+        n = pts.shape[1]
+        pp, ld = _tensor_ptr(pts, PT_NFIELDS, n, "pts")
+        if distr is None:
+            distr = torch.empty((DI_NFIELDS, ld), dtype=torch.float64, device=pts.device)
+        dp, _ = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+        self._check(self.lib.bnsamd_distr_from_m1(self.h, n, ld, pp, dp), "bnsamd_distr_from_m1")
+        return distr
+
+    def equilibrium(self, pts, distr=None):
+        """Writes equilibrium n, J into pts (in place) and returns the
+        equilibrium distribution block."""
+        import torch
+        n = pts.shape[1]
+        pp, ld = _tensor_ptr(pts, PT_NFIELDS, n, "pts")
+        if distr is None:
+            distr = torch.empty((DI_NFIELDS, ld), dtype=torch.float64, device=pts.device)
+        dp, _ = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+        self._check(self.lib.bnsamd_equilibrium(self.h, n, ld, pp, dp), "bnsamd_equilibrium")
+        return distr
+
+    def fdi(self, k: int, y):
+        import torch
+        self._follow_torch_stream()
+        out = torch.empty_like(y)
+        self._check(self.lib.bnsamd_fdi(self.h, int(k), y.numel(), y.data_ptr(), out.data_ptr()),
+                    "bnsamd_fdi")
+        return out

@@ -1,0 +1,834 @@
+// gray_kernel.hip -- fused gray M1 opacity kernel for CDNA4 (gfx950), FP64.
+//
+// One wavefront computes one thermodynamic point
+// (ComputeM1Opacities, m1_opacities.hpp:1062-1313).  Its 64 lanes share the
+// work of that point:
+//   1D   isoenergetic scattering and beta processes: one (species, energy)
+//        item per lane, wave-reduced;
+//   2D   node tables (Fermi integrals F_1..F_5 at eta -+ y_a, distribution
+//        values) staged in LDS, then one pair/bremsstrahlung cell (a <= b)
+//        per lane per round, both orderings (a,b) and (b,a) accumulated;
+//   NEPS one half-cell (i, j, half) per lane per round;
+// and the 16 accumulators of the 2D sums are reduced across the wavefront at
+// the end.  The cell geometry is point-independent (gray_tables.h) and read
+// from a resident table; per-point constants are computed once per wave.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gray_kernel.h"
+#include "gray_math.h"
+
+namespace bns {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Distribution parameters of one species, in the form the kernel evaluates
+// TotalNuF (distribution.hpp:298-315):
+//   f(w) = wt / (e^{(w - mut)/Tt} + 1) + wf bf w^cf e^{-w/Tf}
+struct Spec {
+  double wt, mut, invTt, wf, bf, cf, invTf;
+};
+
+__device__ __forceinline__ double nuf(double w, double lw, const Spec& d) {
+  double arg = (w - d.mut) * d.invTt;
+  double thin = d.bf * exp(fma(d.cf, lw, -w * d.invTf));
+  return fma(d.wt, fermi_of_arg(arg), d.wf * thin);
+}
+
+__device__ __forceinline__ Spec pick(const Spec* D, int s) {
+  Spec r = D[0];
+  if (s == 1) r = D[1];
+  if (s == 2) r = D[2];
+  if (s == 3) r = D[3];
+  return r;
+}
+
+// NucFrmFac (weak_magnetism.hpp:45-104)
+__device__ __forceinline__ void form_factors(double E, int reac, double& cv, double& ca,
+                                             double& F2) {
+  const double lamp = 1.793, lamn = -1.913;
+  double e = E * c::WMEScale;
+  double tau = 0.5 * (e * e) / (1. + e);
+  double eta = 1. / fma(5.6, tau, 1.);
+  double g = fma(4.97, tau, 1.);
+  double G = 1. / (g * g);
+  double itau = 1. / (1. + tau);
+  double Fp1 = fma(tau, 1. + lamp, 1.) * G * itau;
+  double Fp2 = lamp * G * itau;
+  double Fn1 = tau * lamn * (1. - eta) * G * itau;
+  double Fn2 = lamn * fma(tau, eta, 1.) * G * itau;
+  double c3 = fma(3.53, tau, 1.);
+  double ic3 = 1. / (c3 * c3);
+  const double hw = 0.5 - 2. * c::SinThW2;
+  if (reac == 1) {
+    cv = hw * Fp1 - 0.5 * Fn1;
+    ca = 0.5 * (c::Ga - c::Gs) * ic3;
+    F2 = hw * Fp2 - 0.5 * Fn2;
+  } else if (reac == 2) {
+    cv = hw * Fn1 - 0.5 * Fp1;
+    ca = -0.5 * (c::Ga + c::Gs) * ic3;
+    F2 = hw * Fn2 - 0.5 * Fp2;
+  } else {
+    cv = Fp1 - Fn1;
+    ca = c::Ga * ic3;
+    F2 = Fp2 - Fn2;
+  }
+}
+
+// WMScatt R0 - R1/3 combination of IsoScattNucleon (formulas_beta_iso.hpp:
+// 411-451, weak_magnetism.hpp:165-210) for one nucleon.
+__device__ __forceinline__ double iso_leg(double omega, int reac, bool wm) {
+  double c0 = reac == 1 ? c::IsoC0p : c::IsoC0n;
+  double c1 = reac == 1 ? c::IsoC1p : c::IsoC1n;
+  if (!wm) return c0 - c1 / 3.;
+  double cv, ca, F2;
+  form_factors(omega, reac, cv, ca, F2);
+  double hv = reac == 1 ? c::Hpv : c::Hnv, ha = reac == 1 ? c::Hpa : c::Hna;
+  double h0 = hv * hv + 3. * (ha * ha), h1 = hv * hv - ha * ha;
+  double e = omega * c::WMEScale;
+  double eF = e * F2;
+  double R0 = (cv * cv + 3. * (ca * ca) + 1.5 * (eF * eF) + 2. * (e * e) * cv * F2) / h0;
+  double R1 = (cv * cv - ca * ca - 2. * (eF * eF) - 4. * (e * e) * cv * F2) / h1;
+  return c0 * R0 - c1 * R1 / 3.;
+}
+
+// WMAbsEm (weak_magnetism.hpp:123-160): R (neutrino) or Rbar (antineutrino)
+__device__ __forceinline__ double wm_abs(double omega, int s) {
+  double cv, ca, F2;
+  form_factors(omega, 3, cv, ca, F2);
+  double e = omega * c::WMEScale;
+  double a = fma(4. / 3., e, 1.);
+  double tmp1 = (cv * cv) * (1. + 4. * e + (16. / 3.) * (e * e)) + 3. * ca * ca * (a * a) +
+                (8. / 3.) * cv * F2 * (e * e) + (5. / 3.) * (e * e) * fma(2. / 5., e, 1.) * (F2 * F2);
+  double tmp2 = 4. * (cv + F2) * ca * e * a;
+  double b = fma(2., e, 1.);
+  double tmp3 = (c::Gv * c::Gv + 3. * (c::Ga * c::Ga)) * (b * b * b);
+  return (s == 0 ? tmp1 + tmp2 : tmp1 - tmp2) / tmp3;
+}
+
+// EtaNNAbs (formulas_beta_iso.hpp:64-97)
+__device__ __forceinline__ double eta_nn_abs(double n_in, double n_out, double mu_hat,
+                                             double T) {
+  if (fabs(mu_hat) < 1.0e-02) return n_in;
+  double e = (n_out - n_in) / (safe_exp(-mu_hat / T) - 1.);
+  return e < 0. ? n_in : e;
+}
+
+// EtaNNSc (formulas_beta_iso.hpp:371-395)
+__device__ __forceinline__ double eta_nn_sc(double nb, double T, double yN) {
+  double nN = yN * nb;
+  if (nN <= 0.) return 0.;
+  double eF = c::IsoEF * exp((2. / 3.) * log(nN));
+  double aux = 1.5 * T / eF;
+  return nN * aux / sqrt(1. + aux * aux);
+}
+
+// Per-point, per-channel constants of the HR98 fit that do not depend on
+// the cell (kernel_brem_HR98.hpp:239-338)
+struct HrChan {
+  double weight;  // n_N * s-weights of the channel sum (incl. 28/3 for np)
+  double gamma_half_gb;
+  double gamma_over_T;
+  double sd_pt;   // s_d / (cell factor): 3 (pi/2)^2.5 es^-2.5 f_u
+  double f_pt;    // (1 + es^2)(1 + y^4)
+  double g0;      // c1 y / (0.8 + 0.06 y^1.05) sqrt(es) / (es + 0.2)
+  double h;       // h_brem
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+template <int WAVES_PER_EU>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_f64(
+    GrayKernelArgs A) {
+  __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
+  __shared__ double s_Fp[5][64];  // F_1..F_5 (eta + y_a)
+  __shared__ double s_g[4][64];   // TotalNuF(nu_a, s)
+  __shared__ double s_bf[4][64];  // blocking factor 1 - g (or 1)
+  __shared__ double s_y[64];      // node energies / T, formed as the reference does
+  // wave-uniform per-point data, read back with broadcast LDS loads so that
+  // it occupies neither scalar nor vector registers through the cell loops
+  __shared__ double s_in[64];     // the 21 inputs (+ 28 distribution values)
+  __shared__ Spec s_D[4];
+  __shared__ double s_Fe[2][6], s_fd0[2];
+  __shared__ HrChan s_hc[3];
+
+  const int lane = threadIdx.x;
+  const int64_t p = blockIdx.x;
+  if (p >= A.npts) return;
+  const int64_t ld = A.ld;
+  const double* __restrict__ tb = A.tab;
+  const TableLayout L = A.L;
+  const volatile double* vin = s_in;
+
+  // ---- point inputs: lane f loads field f ----------------------------------
+  if (lane < 21) s_in[lane] = A.pts[lane * ld + p];
+  else if (A.distr && lane < 49) s_in[lane] = A.distr[(lane - 21) * ld + p];
+  __syncthreads();
+  const double T = vin[1];
+  const double mu_e = vin[6];
+
+  // ---- distribution parameters: lane s < 4 builds species s ------------------
+  if (lane < 4) {
+    const int s = lane;
+    Spec d;
+    if (A.distr) {
+      double Tt = vin[21 + 4 + s];
+      d.wt = vin[21 + 0 + s];
+      d.mut = Tt * vin[21 + 8 + s];
+      d.invTt = 1. / Tt;
+      d.wf = vin[21 + 12 + s];
+      d.invTf = 1. / vin[21 + 16 + s];
+      d.cf = vin[21 + 20 + s];
+      d.bf = vin[21 + 24 + s];
+    } else {
+      // CalculateThickParamsFromM1 + CalculateThinParamsFromM1
+      // (distribution.hpp:51-243)
+      double n0 = vin[9 + s], J0 = vin[13 + s], chi = vin[17 + s];
+      double nn_ = n0 > 1e-100 ? n0 : 1e-100;
+      double JJ = J0 > 1e-100 ? J0 : 1e-100;
+      double r = nn_ / JJ;
+      double y = nn_ * (r * r * r) / c::FourPiHc3;
+      double et;
+      if (y < 0.005) {
+        double num = y * (y * (y * (y * (y * (y + 0.19926987701997) + 38865.0149220478) +
+                     14364.6331099737) + 5750.1878570758) + 1120.71610972194) + 1.60356108438235e-8;
+        double den = y * (y * (y * (y * (y * (y + 1.0) + 38840.0743174942) - 99.9009680656931) -
+                     171.874844843596) + 75.7101579899442) + 83.0160130941424;
+        et = log(num / den);
+      } else if (y <= 0.7) {
+        double num = y * (y * (y * (y * (y * (41.3836568203438 * y + 32.5515666786612) -
+                     157.774993512235) + 66.5726772426253) + 14.4883415579211) +
+                     0.315360380575709) + 0.000660414331285249;
+        double den = y * (y * (y * (y * (y * (y + 1.8888797407042) - 5.35488690539183) +
+                     1.94673781342617) + 0.483128792035557) + 0.0113386564109086) +
+                     2.64160073447322e-5;
+        et = num / den - 30.;
+      } else if (y > 0.7 && y < 0.7901234567745267) {
+        double num = y * (y * (y * (y * (y * (3852.81416018959 * y - 5316.18895799799) +
+                     1102.91561586553) + 1.54082262710661e-6) + 1732.89925128741) -
+                     1769.59868329086) + 586.406885304906;
+        double den = y * (y * (y * (y * (y * (y + 255.936658313629) + 9.42360945627147e-5) -
+                     81.2467063138386) - 180.100197053091) - 89.0343496217014) +
+                     143.849128123195;
+        et = exp(num / den);
+      } else {
+        et = 20.;
+      }
+      et = et < 20. ? et : 20.;
+      double F[6];
+      fdi_bundle<2, 3>(et, F);
+      double Tt = F[2] * JJ / (F[3] * nn_);
+      d.wt = 1.5 * (1. - chi);
+      d.mut = Tt * et;
+      d.invTt = 1. / Tt;
+      d.wf = 0.5 * (mul_rn(3., chi) - 1.);
+      double Tnu = JJ / (nn_ * 3.6);
+      d.invTf = 1. / Tnu;
+      d.cf = 0.6;
+      d.bf = nn_ / (c::FourPiHc3 * A.gamma36 * exp(3.6 * log(Tnu)));
+    }
+    s_D[s] = d;
+  }
+
+  // ---- per-point constants --------------------------------------------------
+  const double eta = mu_e / T;
+  {
+    // lanes 0/1 evaluate F_0..F_5 at +eta / -eta; lanes 0..3 publish one
+    // species' distribution parameters each
+    double F[6];
+    fdi_bundle<0, 5>(lane == 0 ? eta : -eta, F);
+    if (lane < 2) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_Fe[lane][k] = F[k];
+      s_fd0[lane] = fermi_of_arg(lane == 0 ? -eta : eta);
+    }
+  }
+  __syncthreads();
+  const volatile Spec* vD = s_D;
+  auto Dget = [&](int s) {
+    Spec r;
+    r.wt = vD[s].wt;
+    r.mut = vD[s].mut;
+    r.invTt = vD[s].invTt;
+    r.wf = vD[s].wf;
+    r.bf = vD[s].bf;
+    r.cf = vD[s].cf;
+    r.invTf = vD[s].invTf;
+    return r;
+  };
+  const volatile double (*vFe)[6] = s_Fe;
+
+  double acc[16];  // [0..3] n_em, [4..7] n_ab, [8..11] e_em, [12..15] e_ab
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.;
+
+  // ======================= 1D: isoenergetic scattering =======================
+  double iso[4] = {0., 0., 0., 0.};
+  const double s_nux = 1.5 * T;
+  double s_iso[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    double ns = vin[9 + s];
+    s_iso[s] = (ns > c::ThresholdN) ? (vin[13 + s] / ns) : s_nux;
+  }
+  const int n1 = L.n1;
+  if (A.flags.use_iso == 1) {
+    const double etaNNp = eta_nn_sc(vin[0], T, vin[2]), etaNNn = eta_nn_sc(vin[0], T, vin[3]);
+    const bool wm = A.params.use_WM_sc != 0;
+    double a4[4] = {0., 0., 0., 0.};
+    for (int it = lane; it < 8 * n1; it += kWave) {
+      int s = it / (2 * n1), r = it - s * 2 * n1, half = r >= n1, i = r - half * n1;
+      double x = tb[L.q1 + Q1_REC * i + Q1_X], w = tb[L.q1 + Q1_REC * i + Q1_W];
+      double ts = s == 0 ? s_iso[0] : s == 1 ? s_iso[1] : s == 2 ? s_iso[2] : s_iso[3];
+      double nu = half ? ts / x : ts * x;
+      double W = half ? w / (x * x) : w;
+      double g = nuf(nu, log(nu), Dget(s));
+      double isc = etaNNp * iso_leg(nu, 1, wm) + etaNNn * iso_leg(nu, 2, wm);
+      double nu2 = nu * nu;
+      double v = W * g * (4. * c::Pi) * nu2 * nu2 * nu * isc;
+      a4[0] += s == 0 ? v : 0.;
+      a4[1] += s == 1 ? v : 0.;
+      a4[2] += s == 2 ? v : 0.;
+      a4[3] += s == 3 ? v : 0.;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) iso[s] = wave_sum(a4[s]) * s_iso[s];
+  }
+
+  // ========================== 1D: beta processes ============================
+  double bne[2] = {0., 0.}, bje[2] = {0., 0.}, bna[2] = {0., 0.}, bja[2] = {0., 0.};
+  double s_beta[2];
+  if (eta > -30. && eta < 30.) {
+    s_beta[0] = T * vFe[0][5] / vFe[0][4];
+    s_beta[1] = T * vFe[1][5] / vFe[1][4];
+  } else if (eta > 30.) {
+    s_beta[0] = T * eta * 0.8333333333333333;
+    s_beta[1] = T * 5.;
+  } else {
+    s_beta[0] = T * 5.;
+    s_beta[1] = T * eta * 0.8333333333333333;
+  }
+  if (A.flags.use_abs_em == 1) {
+    const PointParams& pp = A.params;
+    const double nb = vin[0], yp = vin[2], yn = vin[3], mu_p = vin[4], mu_n = vin[5];
+    double nn_ = nb * yn, np_ = nb * yp;
+    double dU = pp.use_dU ? vin[7] : 0.;
+    double dQ = pp.use_dm_eff ? vin[8] : c::Q;
+    double mu_hat = mu_n - mu_p - dQ;
+    double Qp = dQ + dU, mu_np = mu_hat - dU;
+    double etanp = eta_nn_abs(nn_, np_, mu_np, T);
+    double etapn = eta_nn_abs(np_, nn_, -mu_np, T);
+    double munu = mu_p + mu_e - mu_n;
+    const double m = c::Me, m2 = m * m;
+    double a8[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    for (int it = lane; it < 4 * n1; it += kWave) {
+      int s = it / (2 * n1), r = it - s * 2 * n1, half = r >= n1, i = r - half * n1;
+      double x = tb[L.q1 + Q1_REC * i + Q1_X], w = tb[L.q1 + Q1_REC * i + Q1_W];
+      double ts = s == 0 ? s_beta[0] : s_beta[1];
+      double nu = half ? ts / x : ts * x;
+      double W = half ? w / (x * x) : w;
+      double g = nuf(nu, log(nu), Dget(s));
+      double R = pp.use_WM_ab ? wm_abs(nu, s) : 1.;
+      // AbsOpacitySingleLep (formulas_beta_iso.hpp:138-279) for species s:
+      // captured lepton energy Ec (e- for nue, e+ for anue), decay lepton Ed
+      double sg = s == 0 ? 1. : -1.;
+      double Ec = nu + sg * Qp, Ed = -Ec;
+      double muc = sg * mu_e;
+      double fc = fermi_of_arg((Ec - muc) / T);
+      double fdd = fermi_of_arg((Ed + muc) / T);
+      double cap = 0., dec = 0.;
+      if (Ec - m >= 0.) cap = Ec * Ec * sqrt(1. - m2 / (Ec * Ec)) * R;
+      if (pp.use_decay && (Ed - m >= 0.)) dec = Ed * Ed * sqrt(1. - m2 / (Ed * Ed));
+      double em = c::BetaConst * (s == 0 ? etapn : etanp) * (cap * fc + dec * (1. - fdd));
+      double ab = em * safe_exp((nu - sg * munu) / T);
+      if (A.stim == 1) ab = ab + em;
+      double nu2 = nu * nu;
+      double ve = W * nu2 * em, va = W * nu2 * g * ab;
+      a8[0] += s == 0 ? ve : 0.;
+      a8[1] += s == 1 ? ve : 0.;
+      a8[2] += s == 0 ? ve * nu : 0.;
+      a8[3] += s == 1 ? ve * nu : 0.;
+      a8[4] += s == 0 ? va : 0.;
+      a8[5] += s == 1 ? va : 0.;
+      a8[6] += s == 0 ? va * nu : 0.;
+      a8[7] += s == 1 ? va * nu : 0.;
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      bne[s] = wave_sum(a8[0 + s]) * s_beta[s];
+      bje[s] = wave_sum(a8[2 + s]) * s_beta[s];
+      bna[s] = wave_sum(a8[4 + s]) * s_beta[s];
+      bja[s] = wave_sum(a8[6 + s]) * s_beta[s];
+    }
+  }
+
+  const bool blocking = A.params.neglect_blocking == 0;
+  const bool stim = A.stim == 1;
+
+  // ========================= 2D: pair + bremsstrahlung =======================
+  const bool do_pair = A.flags.use_pair == 1;
+  const bool do_brem = A.flags.use_brem == 1 && A.params.brem_impl != 2;
+  const int N = L.N;
+  if (A.flags.use_pair == 1 || A.flags.use_brem == 1) {
+    // node tables.  The node energy nu_a = t x_a or t / x_a with t = 2.182 T
+    // and y_a = nu_a / T are formed with the reference's roundings
+    // (m1_opacities.hpp:783-787, kernel_pair.hpp:142-143) so that thresholds
+    // such as eta - y - z > 200 decide as they do there.
+    const double tpair = c::PairScale * T;
+    auto node_y = [&](int a) {
+      double xa = tb[L.nd + ND_REC * a + ND_X];
+      double nua = tb[L.nd + ND_REC * a + ND_INV] != 0. ? tpair / xa : mul_rn(tpair, xa);
+      return nua / T;
+    };
+    for (int task = lane; task < 3 * N; task += kWave) {
+      if (task < 2 * N) {
+        if (do_pair) {
+          int a = task < N ? task : task - N;
+          double ya = node_y(a);
+          double F[6];
+          fdi_bundle<1, 5>(task < N ? eta - ya : eta + ya, F);
+          if (task < N) {
+#pragma unroll
+            for (int k = 1; k <= 5; ++k) s_Fm[k - 1][a] = F[k];
+          } else {
+#pragma unroll
+            for (int k = 1; k <= 5; ++k) s_Fp[k - 1][a] = F[k];
+          }
+        }
+      } else {
+        int a = task - 2 * N;
+        double xa = tb[L.nd + ND_REC * a + ND_X];
+        double nu = tb[L.nd + ND_REC * a + ND_INV] != 0. ? tpair / xa : mul_rn(tpair, xa);
+        s_y[a] = nu / T;
+        double lw = log(nu);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          double g = nuf(nu, lw, Dget(s));
+          s_g[s][a] = g;
+          s_bf[s][a] = blocking ? 1. - g : 1.;
+        }
+      }
+    }
+    __syncthreads();
+
+    // per-point bremsstrahlung constants
+    double brt_pre = 0.;
+    double hr_y = 0., hr_emy = 0., hr_p = 1., hr_pinv = 1., hr_medium = 1.;
+    if (do_brem && A.params.brem_impl == 1) {
+      // QBrem_BRT06 and the prefactor of BremKernelsBRT06
+      // (kernel_brem_BRT06.hpp:24-59)
+      const double nb = vin[0], yp = vin[2], yn = vin[3];
+      double rho = nb * c::Mb;
+      double q = 2.0778e+02 * 0.5 * c::MeV * (yn * yn + yp * yp + (28. / 3.) * yn * yp) *
+                 (rho * rho) * exp(5.5 * log(T));
+      double T2 = T * T, T7 = T2 * T2 * T2 * T;
+      brt_pre = c::HC6FourPiSq * c::BremC4BRT06 * (q / T7);
+    } else if (do_brem) {
+      // BremKernelS / BremKernelG / single channels, the parts that do not
+      // depend on x (kernel_brem_HR98.hpp:33-338)
+      double T10 = T * 0.1;
+      double y = 1.94 / T10;
+      hr_y = y;
+      hr_emy = safe_exp(-y / 12.);
+      hr_p = 0.67 + 0.18 * exp(0.4 * log(y));
+      hr_pinv = -1. / hr_p;
+      const double nb = vin[0], yp = vin[2], yn = vin[3];
+      double xmean = sqrt(yn * yp);
+      double nN[3] = {nb * yn, nb * yp, nb * xmean};
+      double mN[3] = {c::MnGrams, c::MpGrams, c::MAvgGrams};
+      double cw[3] = {1., 1., 28. / 3.};
+      double ye = y > c::BremMin ? y : c::BremMin;
+      double y4 = ye * ye * ye * ye;
+      double y105 = exp(1.05 * log(ye));
+      {
+        const int ch = lane < 3 ? lane : 0;
+        double nNc = ch == 0 ? nN[0] : ch == 1 ? nN[1] : nN[2];
+        double mNc = ch == 0 ? mN[0] : ch == 1 ? mN[1] : mN[2];
+        double cwc = ch == 2 ? cw[2] : 1.;
+        double es = exp((2. / 3.) * log(3. * c::PiSquared * nNc)) * c::BremAux1 /
+                    (2. * mNc * T);
+        double gamma = 1.63 * exp(1.5 * log(es)) * T10;
+        if (!(nNc > 0.)) {  // pow(0, 2/3) = 0 in the reference
+          es = 0.;
+          gamma = 0.;
+        }
+        // BremKernelG (kernel_brem_HR98.hpp:165-236), clamped arguments
+        double yg = ye;
+        double eg = es > c::BremMin ? es : c::BremMin;
+        double y2 = yg * yg, ei = 1. / eg, a1d = 25. * y2 + 1.;
+        double alpha1 = (0.5 + ei) / (1. + ei) * (1. / a1d) + (0.5 + eg / 15.6) * 25. * y2 / a1d;
+        double alpha2 = (0.63 + 0.04 * exp(1.45 * log(eg))) / (1. + 0.02 * exp(2.5 * log(eg)));
+        double eg15 = exp(1.5 * log(eg));
+        double alpha3 = 1.2 * safe_exp(0.6 * eg - 0.4 * eg15);
+        double p1 = (1.8 + 0.45 * eg) / (1. + 0.15 * eg15);
+        double p2 = 2.3 - 0.05 * eg / (1. + 0.025 * eg);
+        double ly = log(yg);
+        double gb = (alpha1 + alpha2 * exp(p1 * ly)) /
+                    (1. + alpha3 * exp(p2 * ly) + alpha2 * exp((p1 + 2.) * ly) / 13.75);
+        // BremKernelS, x-independent factors
+        double u = sqrt(ye / (2. * eg)) + 1.e-10;
+        double u2 = u * u;
+        double uarg = u2 / (2. * sqrt(2. * u2 + 4.));
+        double fu = (1. - 5. * u * atan(2. / u) / 6. + u2 / (3. * (u2 + 4.)) +
+                     atan(1. / uarg) * uarg / 3.);
+        fu = fabs(fu) < 1e-14 ? 1e-14 : fu;
+        if (lane < 3) {
+          s_hc[ch].sd_pt = 3. * c::PiHalfToFiveHalves * exp(-2.5 * log(eg)) * fu;
+          s_hc[ch].f_pt = (1. + eg * eg) * (1. + y4);
+          s_hc[ch].g0 = 0.0044 * ye / (0.8 + 0.06 * y105) * sqrt(eg) / (eg + 0.2);
+          s_hc[ch].h = 0.1 * eg / (2.39 + 0.1 * exp(1.1 * log(eg)));
+          s_hc[ch].gamma_half_gb = 0.5 * gamma * gb;
+          s_hc[ch].gamma_over_T = gamma / T;
+          s_hc[ch].weight = cwc * nNc;
+        }
+      }
+      if (A.params.use_NN_medium_corr) {
+        double cc = 1. + cbrt(vin[0] / c::SaturationN) / 3.;
+        double c2 = cc * cc;
+        hr_medium = 1. / (c2 * c2 * c2);
+      }
+    }
+
+    __syncthreads();
+    const double a1e = c::PairA1E * c::PairA1E, a2e = c::PairA2E * c::PairA2E;
+    const double a1x = c::PairA1X * c::PairA1X, a2x = c::PairA2X * c::PairA2X;
+    const double pairT2 = c::PairPhiC * (T * T);
+    for (int cc = lane; cc < L.P; cc += kWave) {
+      const double* __restrict__ cr = tb + L.cl + CL_REC * cc;
+      int ab = int(cr[CL_AB]);
+      int a = ab >> 8, b = ab & 255;
+      double ya = s_y[a], yb = s_y[b];
+      double Kem[4] = {0., 0., 0., 0.}, Kab[4] = {0., 0., 0., 0.};
+      if (do_pair) {
+        double psi0 = 0., psi1 = 0.;
+        double emyz = (eta - ya) - yb;
+        if (!(emyz > 200.)) {
+          // PairPsi (kernel_pair.hpp:16-120)
+          double Fm3[6], Fp3[6];
+          fdi_bundle<3, 5>(emyz, Fm3);
+          fdi_bundle<3, 5>((eta + ya) + yb, Fp3);
+          double F1emy = s_Fm[0][a], F1emz = s_Fm[0][b], F1epy = s_Fp[0][a], F1epz = s_Fp[0][b];
+          double F2emy = s_Fm[1][a], F2emz = s_Fm[1][b], F2epy = s_Fp[1][a], F2epz = s_Fp[1][b];
+          double F3emy = s_Fm[2][a], F3emz = s_Fm[2][b], F3epy = s_Fp[2][a], F3epz = s_Fp[2][b];
+          double F4emy = s_Fm[3][a], F4emz = s_Fm[3][b], F4epy = s_Fp[3][a], F4epz = s_Fp[3][b];
+          double F5emy = s_Fm[4][a], F5emz = s_Fm[4][b], F5epy = s_Fp[4][a], F5epz = s_Fp[4][b];
+          double y = ya, z = yb;
+          double x0 = 20. * F4emy, x1 = 20. * F4epz;
+          double x2 = 120. * (F2emy - F2epz);
+          double x3 = 40. * (F3epz - F3emy);
+          double x4 = 40. * vFe[0][3];
+          double x5 = 40. * Fm3[3] - x4;
+          double x6 = -20. * vFe[0][4];
+          double x7 = 20. * Fm3[4] + x6;
+          double x8 = x4 - 40. * Fp3[3];
+          double x9 = 20. * Fp3[4] + x6;
+          double x10 = 4. * (((((Fm3[5] - F5emy) - F5emz) + F5epy) - Fp3[5]) + F5epz);
+          double x11 = 20. * F4epy, x12 = 20. * F4emz;
+          double x13 = 40. * (F3epy - F3emz);
+          double x14 = 120. * (F2emz - F2epy);
+          double yz = y * z;
+          double aux = 15. * (yz * yz);
+          psi0 = x10 +
+                 y * (-x0 + x1 + x7 +
+                      y * (x3 + x5 + z * (x2 + z * (120. * (F1epz - F1emy)))) +
+                      z * (80. * (F3emy - F3epz) - x2 * z)) +
+                 z * (x0 - x1 + x9 + z * (x3 + x8));
+          psi1 = x10 + y * (-x11 + x12 + x9 + y * (x13 + x8)) +
+                 z * (x11 - x12 + x7 + y * (80. * (F3emz - F3epy) - x14 * y) +
+                      z * (x13 + x5 + y * (x14 + y * (120. * (F1epy - F1emz)))));
+          double ia = 1. / aux;
+          psi0 *= ia;
+          psi1 *= ia;
+        }
+        // PairPhi + PairKernels (kernel_pair.hpp:135-198): em = phi/2.  Past
+        // the SafeExp clamp (y + z > 700) phi is subnormal; it is rounded and
+        // then halved in the reference's order, because the absorption kernel
+        // multiplies it by e^700 afterwards.
+        double aux = pairT2 * cr[CL_PAUX];
+        double db = cr[CL_DB];
+        Kem[0] = 0.5 * ((a1e * psi0 + a2e * psi1) * aux);
+        Kem[1] = 0.5 * ((a1e * psi1 + a2e * psi0) * aux);
+        Kem[2] = 0.5 * ((a1x * psi0 + a2x * psi1) * aux);
+        Kem[3] = 0.5 * ((a1x * psi1 + a2x * psi0) * aux);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) Kab[s] = db * Kem[s];
+      }
+      double bem = 0., bab = 0.;
+      if (do_brem) {
+        if (A.params.brem_impl == 1) {
+          bem = brt_pre * cr[CL_K1EM];
+          bab = brt_pre * cr[CL_K1AB];
+        } else {
+          // BremKernelS at x = (w+w')/T for the three channels
+          double x = cr[CL_S];
+          x = x > c::BremMin ? x : c::BremMin;
+          double y = hr_y > c::BremMin ? hr_y : c::BremMin;
+          double snum = 2. * c::SqrtPi * exp(1.5 * log(x + 2. - hr_emy)) *
+                        (x * x + 2. * x * y + 5. * (y * y) / 3. + 1.);
+          double bb = c::Pi2OneEighth + x + y;
+          double b2 = bb * bb;
+          double snd = snum / (c::SqrtPi + b2 * b2);
+          double lsnd = log(snd), lsdx = cr[CL_LSD];
+          double pnd = exp(-hr_p * lsnd);
+          double px11 = cr[CL_X11], px093 = cr[CL_X093];
+          double px12 = cr[CL_X12], px28 = cr[CL_X28];
+          double xm = x - 1.2;
+          double fx = 3. + xm * xm + cr[CL_XM4];
+          double tot = 0.;
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            const volatile HrChan& h = s_hc[ch];
+            double sd = exp(-hr_p * (log(h.sd_pt) + lsdx));
+            double sbase = exp(hr_pinv * log(pnd + sd));
+            double fb = 1. + 1. / (fx * h.f_pt);
+            double gbm = 1. - px11 * h.g0;
+            double cb = 1.1 * px11 * h.h / (2.3 + h.h * px093 + 0.0001 * px12) * 30. /
+                        (30. + 0.005 * px28);
+            double sb = sbase * fb * (1. + cb * gbm);
+            double gx = h.gamma_half_gb;
+            double sabs = h.gamma_over_T / (x * x + gx * gx) * sb;
+            tot += h.weight * sabs;
+          }
+          bab = 3. * c::BremConst * tot * hr_medium;
+          bem = bab * cr[CL_EMX];
+        }
+      }
+      // weights: blocking / stimulated absorption (m1_opacities.hpp:577-758)
+      double wab = tb[L.nd + ND_REC * a + ND_WQ] * tb[L.nd + ND_REC * b + ND_WQ];
+#pragma unroll
+      for (int ord = 0; ord < 2; ++ord) {
+        if (ord == 1 && a == b) break;
+        int u = ord == 0 ? a : b, v = ord == 0 ? b : a;  // nu at u, nubar at v
+        double yu = ord == 0 ? ya : yb;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          int ps = s ^ 1;
+          double ke = (ord == 0 ? Kem[s] : Kem[ps]) + bem;
+          double ka = (ord == 0 ? Kab[s] : Kab[ps]) + bab;
+          double gu = s_g[s][u];
+          double eb = ke * s_bf[ps][v];
+          double e_ent, a_ent;
+          if (stim) {
+            e_ent = eb;
+            a_ent = gu * fma(ka, s_g[ps][v], eb);
+          } else {
+            e_ent = (1. - gu) * eb;
+            a_ent = gu * ka * s_g[ps][v];
+          }
+          acc[s] = fma(wab, e_ent, acc[s]);
+          acc[4 + s] = fma(wab, a_ent, acc[4 + s]);
+          acc[8 + s] = fma(wab * yu, e_ent, acc[8 + s]);
+          acc[12 + s] = fma(wab * yu, a_ent, acc[12 + s]);
+        }
+      }
+    }
+  }
+
+  // ====================== 2D: inelastic scattering on e-+ ===================
+  if (A.flags.use_inelastic_scatt == 1) {
+    const double pre = c::NEPSConst * (T * T);
+    const double bpl = c::NEPSBPlus, bze = c::NEPSBZero, bmi = c::NEPSBMinus;
+    // t = 4 s_neps with s_neps = 2.182 T (m1_opacities.hpp:1092, 1167)
+    const double ht = 0.5 * (4. * (c::PairScale * T));
+    for (int h = lane; h < L.Q; h += kWave) {
+      const double* __restrict__ hr = tb + L.np + NP_REC * h;
+      // nu, nubar and w = nu / T with the reference's roundings
+      // (m1_opacities.hpp:910-912, 982-984; kernel_nes.hpp:144-145) so that
+      // the Taylor-branch tests of NESKernels / NPSKernels decide identically
+      double xi = hr[NP_XI];
+      double nu, nub;
+      if (hr[NP_HALF] == 0.) {
+        double hx = mul_rn(ht, xi);
+        nu = mul_rn(hx, hr[NP_OMX]);
+        nub = mul_rn(hx, hr[NP_OPX]);
+      } else {
+        nu = mul_rn(ht, hr[NP_OMX]) / xi;
+        nub = mul_rn(ht, hr[NP_OPX]) / xi;
+      }
+      double w = nu / T, wp = nub / T;
+      double ad = fabs(w - wp);
+      double xm = wp, ym = w;  // w < w' on the whole grid
+      if (w > wp) { xm = w; ym = wp; }
+      double sgn = hr[NP_SIGN];
+      double gn[4], gb[4];
+      {
+        double l1 = log(nu), l2 = log(nub);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          Spec d = Dget(s);
+          gn[s] = nuf(nu, l1, d);
+          gb[s] = nuf(nub, l2, d);
+        }
+      }
+      double out[4] = {0., 0., 0., 0.};
+#pragma unroll
+      for (int lep = 0; lep < 2; ++lep) {
+        const double el = lep == 0 ? eta : -eta;
+        // couplings (b1, b2): NES (B+,B0),(B0,B+),(B-,B0),(B0,B-); NPS swaps
+        double b1[4], b2[4];
+        b1[0] = lep == 0 ? bpl : bze;  b2[0] = lep == 0 ? bze : bpl;
+        b1[1] = lep == 0 ? bze : bpl;  b2[1] = lep == 0 ? bpl : bze;
+        b1[2] = lep == 0 ? bmi : bze;  b2[2] = lep == 0 ? bze : bmi;
+        b1[3] = lep == 0 ? bze : bmi;  b2[3] = lep == 0 ? bmi : bze;
+        if (ym > el * 1e-3) {
+          // MezzacappaIntOut (kernel_nes.hpp:50-78) with the differences of
+          // ComputeFDIForInelastic (:29-44)
+          double A5[6], B5[6], C3[6];
+          fdi_bundle<1, 5>(el - wp, A5);
+          fdi_bundle<1, 5>(el - w, B5);
+          fdi_bundle<3, 5>(el - ad, C3);
+          double fw0 = A5[1] - B5[1], fw1 = A5[2] - B5[2], fw2 = A5[3] - B5[3];
+          double fw3 = A5[4] - B5[4], fw4 = A5[5] - B5[5];
+          double fa0 = vFe[lep][3] - C3[3], fa1 = vFe[lep][4] - C3[4];
+          double fa2 = vFe[lep][5] - C3[5];
+          double ws = w + wp;
+          double inv = hr[NP_INVDEN];
+          double q4 = fw3 + 2. * (ws * fw2 + 3. * w * wp * fw1);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            double t1 = (b1[s] + b2[s]) * (sgn * fa2 - fw4) * 0.2;
+            double t2 = b1[s] * ws * q4;
+            double t3 = sgn * ((b1[s] * xm - b2[s] * ym) * fa1 +
+                               2. * (b1[s] * xm * xm + b2[s] * ym * ym) * fa0);
+            out[s] = fma(pre, (t1 - t2 + t3) * inv - 6. * b1[s] * fw0, out[s]);
+          }
+        } else if (xm > el * 1e-3) {
+          // MezzacappaIntOneEnergy (kernel_nes.hpp:82-107)
+          double D3[6];
+          fdi_bundle<0, 2>(el - xm, D3);
+          double f0 = D3[2] - vFe[lep][2], f1 = D3[1], f2 = vFe[lep][1];
+          double f3 = D3[0] + ym * fermi_of_arg(xm - el) / 6.;
+          double f4 = vFe[lep][0] - ym * s_fd0[lep] / 6.;
+          double ix2 = 1. / (xm * xm);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            double B1 = b1[s], B2 = b2[s];
+            double S = 2. * (B1 + B2) * f0 + (B1 * (ym + 4. * xm) + 3. * B2 * ym) * f1 +
+                       (B1 * (3. * ym - 4. * xm) + B2 * ym) * f2 +
+                       ((B1 + 6. * B2) * ym * ym * 0.2 + B1 * xm * ym + 2. * B1 * xm * xm) * f3 -
+                       ((6. * B1 + B2) * ym * ym * 0.2 - 3. * B1 * xm * ym + 2. * B1 * xm * xm) * f4;
+            out[s] = fma(pre, -sgn * ym * S * ix2, out[s]);
+          }
+        } else {
+          // MezzacappaIntTwoEnergies (kernel_nes.hpp:111-132)
+          double f0 = s_fd0[lep], f1 = vFe[lep][0], f2 = vFe[lep][1];
+          double ix2 = 1. / (xm * xm);
+          double pol = (11. * ym * ym - 25. * xm * ym + 20. * xm * xm) / 30.;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            double S = (b1[s] + b2[s]) * (4. * f2 + f0 * pol) + (b1[s] - b2[s]) * (2. * xm - ym) * f1;
+            out[s] = fma(pre, ym * (w - wp) * S * ix2, out[s]);
+          }
+        }
+      }
+      // in/out kernels for (nu, nubar) and the swapped pair: the swapped
+      // kernel is the negated out-integral with NEPSExpFunc arguments
+      // exchanged, i.e. em2 = ab1, ab2 = em1 (kernel_nes.hpp:229-234)
+      double ef = hr[NP_EF], efx = hr[NP_EFX];
+      double wq = hr[NP_WQ];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        double em1 = -out[s] * efx, ab1 = out[s] * ef;
+        double bfn = blocking ? 1. - gn[s] : 1., bfb = blocking ? 1. - gb[s] : 1.;
+        double te1 = em1 * gb[s], ta1 = ab1 * bfb;
+        double te2 = ab1 * gn[s], ta2 = em1 * bfn;
+        double Ae, Aa, Be, Ba;
+        if (stim) {
+          Ae = te1;
+          Aa = gn[s] * (te1 + ta1);
+          Be = te2;
+          Ba = gb[s] * (te2 + ta2);
+        } else {
+          Ae = (1. - gn[s]) * te1;
+          Aa = gn[s] * ta1;
+          Be = (1. - gb[s]) * te2;
+          Ba = gb[s] * ta2;
+        }
+        acc[s] = fma(wq, Ae + Be, acc[s]);
+        acc[4 + s] = fma(wq, Aa + Ba, acc[4 + s]);
+        acc[8 + s] = fma(wq, fma(w, Ae, wp * Be), acc[8 + s]);
+        acc[12 + s] = fma(wq, fma(w, Aa, wp * Ba), acc[12 + s]);
+      }
+    }
+  }
+
+  // ============================ reduce + assemble ============================
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = wave_sum(acc[i]);
+  if (lane == 0) {
+    const double tp = c::PairScale * T;
+    const double T2 = T * T;
+    const double sN = tp * tp * (T2 * T2);  // t^2 T^4
+    const double sE = sN * T;               // energy weights are y = nu / T
+    double n2d[8], e2d[8];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      n2d[s] = acc[s] * sN;
+      n2d[4 + s] = acc[4 + s] * sN;
+      e2d[s] = acc[8 + s] * sE;
+      e2d[4 + s] = acc[12 + s] * sE;
+    }
+    const double F = c::FourPiHc3, F2 = c::FourPiHc3Sqr, cl = c::Clight;
+    double o[20];
+#pragma unroll
+    for (int i = 0; i < 20; ++i) o[i] = 0.;
+    // m1_opacities.hpp:1187-1292
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      o[0 + s] = F * (F * n2d[s] + bne[s]);
+      o[8 + s] = F * (F * e2d[s] + bje[s]);
+      const double ns = vin[9 + s], Js = vin[13 + s];
+      if (ns > c::ThresholdN) o[4 + s] = F / (cl * ns) * (F * n2d[4 + s] + bna[s]);
+      if (Js > c::ThresholdJ) {
+        double ka = F / (cl * Js) * (F * e2d[4 + s] + bja[s]);
+        o[12 + s] = (s == 0 && ns == 0.) ? 0. : ka;
+        o[16 + s] = F / (cl * Js) * iso[s];
+      }
+    }
+#pragma unroll
+    for (int s = 2; s < 4; ++s) {
+      o[0 + s] = F2 * n2d[s];
+      o[8 + s] = F2 * e2d[s];
+      const double ns = vin[9 + s], Js = vin[13 + s];
+      if (ns > c::ThresholdN) {
+        double k0 = F2 / (cl * ns) * n2d[4 + s];
+        o[4 + s] = (s == 3 && ns == 0.) ? 0. : k0;
+      }
+      if (Js > c::ThresholdJ) {
+        o[12 + s] = F2 / (cl * Js) * e2d[4 + s];
+        o[16 + s] = F / (cl * Js) * iso[s];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 20; ++i) A.out[i * ld + p] = o[i];
+  }
+}
+
+}  // namespace bns
+
+#ifndef BNS_GRAY_WAVES_PER_EU
+#define BNS_GRAY_WAVES_PER_EU 2
+#endif
+namespace bns {
+hipError_t launch_gray_m1_f64(const GrayKernelArgs& a, hipStream_t stream) {
+  if (a.npts <= 0) return hipSuccess;
+  dim3 grid(static_cast<unsigned>(a.npts)), block(64);
+  hipLaunchKernelGGL(k_gray_m1_f64<BNS_GRAY_WAVES_PER_EU>, grid, block, 0, stream, a);
+  return hipGetLastError();
+}
+}  // namespace bns
+
+namespace bns {
+hipError_t launch_gray_m1_mixed(const GrayKernelArgs&, hipStream_t) { return hipErrorNotSupported; }
+}  // namespace bns

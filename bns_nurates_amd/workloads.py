@@ -9,8 +9,8 @@ the workloads define their own, in closed form:
   (3 pi^2 (hbar c)^3)``, solved exactly (hyperbolic form of Cardano);
 * neutron / proton chemical potentials from non-relativistic ideal Fermi
   gases, ``n_N = 2 (m_N T / (2 pi (hbar c)^2))^{3/2} F_{1/2}(eta_N)``
-  (normalised F), inverted by bisection on the Bednarczyk & Bednarczyk
-  (1978) closed-form approximation of F_{1/2}; ``mu_N = m_N + T eta_N``
+  (normalised F), inverted by Newton iteration on the Bednarczyk &
+  Bednarczyk (1978) closed-form approximation of F_{1/2}; ``mu_N = m_N + T eta_N``
   (rest mass included, ``bns_nurates.hpp:287``);
 * neutrino moments at equilibrium (``NuEquilibriumParams`` +
   ``ComputeM1DensitiesEq``, ``distribution.hpp:253-290,447-473``) times
@@ -80,41 +80,76 @@ def _mu_e_massless(ne: np.ndarray, T: np.ndarray) -> np.ndarray:
     return a * np.sinh(np.arcsinh(arg) / 3.0)
 
 
-def _f12_norm(eta: np.ndarray) -> np.ndarray:
-    """Normalised F_{1/2} (2/sqrt(pi) int sqrt(x)/(e^{x-eta}+1)), closed-form
-    approximation of Bednarczyk & Bednarczyk, Phys. Lett. A 64 (1978) 409."""
-    nu = eta ** 4 + 50.0 + 33.6 * eta * (1.0 - 0.68 * np.exp(-0.17 * (eta + 1.0) ** 2))
-    return 1.0 / (np.exp(-eta) + 0.75 * np.sqrt(np.pi) * nu ** (-0.375))
+def _f12_norm(eta: np.ndarray):
+    """Normalised F_{1/2} (2/sqrt(pi) int sqrt(x)/(e^{x-eta}+1)) and its
+    eta-derivative, closed-form approximation of Bednarczyk & Bednarczyk,
+    Phys. Lett. A 64 (1978) 409."""
+    g = np.exp(-0.17 * (eta + 1.0) ** 2)
+    nu = eta ** 4 + 50.0 + 33.6 * eta * (1.0 - 0.68 * g)
+    dnu = 4.0 * eta ** 3 + 33.6 * (1.0 - 0.68 * g) + 33.6 * eta * 0.68 * 0.34 * (eta + 1.0) * g
+    c = 0.75 * np.sqrt(np.pi)
+    e = np.exp(-np.minimum(eta, 700.0))
+    den = e + c * nu ** (-0.375)
+    dden = -e - 0.375 * c * nu ** (-1.375) * dnu
+    return 1.0 / den, -dden / den ** 2
 
 
 def _eta_ideal_nucleon(n: np.ndarray, T: np.ndarray, m: float) -> np.ndarray:
+    """Degeneracy of an ideal non-relativistic nucleon gas of density n:
+    Newton iteration on log F_{1/2}(eta) = log(n / n_Q)."""
     nq = 2.0 * (m * T / (2.0 * np.pi * HBARC ** 2)) ** 1.5
-    u = n / nq
-    lo = np.full_like(u, -200.0)
-    hi = np.maximum(np.log(np.maximum(u, 1e-300)) + 5.0, (1.5 * u) ** (2.0 / 3.0) + 5.0)
-    for _ in range(80):
-        mid = 0.5 * (lo + hi)
-        f = _f12_norm(mid) - u
-        lo = np.where(f < 0.0, mid, lo)
-        hi = np.where(f < 0.0, hi, mid)
-    return 0.5 * (lo + hi)
+    lu = np.log(n / nq)
+    u = np.exp(np.minimum(lu, 700.0))
+    eta = np.where(lu < 0.0, lu, np.maximum(lu, (0.75 * np.sqrt(np.pi) * u) ** (2.0 / 3.0)))
+    for _ in range(60):
+        f, df = _f12_norm(eta)
+        step = np.clip((np.log(f) - lu) * f / df, -20.0, 20.0)
+        eta = eta - step
+        if np.max(np.abs(step) / np.maximum(1.0, np.abs(eta))) < 1e-14:
+            break
+    return eta
 
 
-def _fdi_quad(k: int, eta: np.ndarray, nodes: int = 64) -> np.ndarray:
-    """Complete Fermi integral F_k(eta) = int_0^inf x^k / (e^{x-eta}+1) dx by
-    two-segment Gauss-Legendre ([0, max(eta,0)] and 60 units beyond)."""
-    x, w = np.polynomial.legendre.leggauss(nodes)
-    x = 0.5 * (x + 1.0)
-    w = 0.5 * w
-    a = np.maximum(eta, 0.0)[:, None]
-    out = np.zeros(eta.shape, dtype=np.float64)
-    for lo, hi in ((0.0 * a, a), (a, a + 60.0)):
-        L = hi - lo
-        xs = lo + L * x[None, :]
-        z = np.clip(xs - eta[:, None], -700.0, 700.0)
-        f = xs ** k / (np.exp(z) + 1.0)
-        out += (L * (f * w[None, :])).sum(axis=1)
-    return out
+# Cohen, Rodriguez Villegas & Zagier (Exp. Math. 9 (2000) 3), algorithm 1:
+# weights that sum an alternating series sum_{m>=0} (-1)^m a_m of a moment
+# sequence a_m to ~5.8^-M relative accuracy with M terms.
+def _crvz_weights(M: int = 24) -> np.ndarray:
+    d = (3.0 + np.sqrt(8.0)) ** M
+    d = 0.5 * (d + 1.0 / d)
+    b, c = -1.0, -d
+    w = np.zeros(M)
+    for k in range(M):
+        c = b - c
+        w[k] = c / d          # coefficient of (-1)^k a_k
+        b = (k + M) * (k - M) * b / ((k + 0.5) * (k + 1))
+    return w
+
+
+_CRVZ = _crvz_weights()
+
+
+def _fdi_int(k: int, eta: np.ndarray) -> np.ndarray:
+    """Complete Fermi integral F_k(eta) = int_0^inf x^k / (e^{x-eta}+1) dx,
+    k >= 1, for all eta: for eta <= 0 the alternating series
+    k! sum_{m>=1} (-1)^{m+1} e^{m eta} / m^{k+1} (accelerated); for eta > 0
+    the reflection F_k(eta) = (-1)^k F_k(-eta) + P_k(eta) with the exact
+    polynomial P_k of the polylogarithm inversion formula."""
+    from math import factorial
+    from scipy.special import zeta
+    a = -np.abs(eta)
+    x = np.exp(a)
+    M = _CRVZ.size
+    s = np.zeros_like(eta)
+    xm = np.ones_like(eta)
+    for m in range(M):
+        xm = xm * x
+        s += _CRVZ[m] * xm / float(m + 1) ** (k + 1)
+    f_neg = factorial(k) * s
+    p = eta ** (k + 1) / (k + 1)
+    for j in range(1, (k + 1) // 2 + 1):
+        p = p + 2.0 * factorial(k) / factorial(k + 1 - 2 * j) * (1.0 - 2.0 ** (1 - 2 * j)) * \
+            zeta(2 * j) * eta ** (k + 1 - 2 * j)
+    return np.where(eta > 0.0, (-1) ** k * f_neg + p, f_neg)
 
 
 def make_points(npts: int, seed: int = 0, rho_range=(1e6, 1e15), T_range=(0.5, 50.0),
@@ -144,8 +179,8 @@ def make_points(npts: int, seed: int = 0, rho_range=(1e6, 1e15), T_range=(0.5, 5
     pts[P_DU] = 0.0
     pts[P_DMEFF] = Q_NP
     for s in range(3):
-        pts[P_N0 + s] = npre * _fdi_quad(2, etas[s]) * fac[0, s]
-        pts[P_J0 + s] = jpre * _fdi_quad(3, etas[s]) * fac[1, s]
+        pts[P_N0 + s] = npre * _fdi_int(2, etas[s]) * fac[0, s]
+        pts[P_J0 + s] = jpre * _fdi_int(3, etas[s]) * fac[1, s]
         pts[P_CHI0 + s] = chi[s]
     pts[P_N0 + 3] = pts[P_N0 + 2]
     pts[P_J0 + 3] = pts[P_J0 + 2]

@@ -1,0 +1,144 @@
+/* bns_nurates_amd.h -- C ABI of the MI355X gray M1 neutrino-opacity library.
+ *
+ * Drop-in boundary for the gray (energy-integrated) M1 opacity path of
+ * bns_nurates.  Every entry point is plain C: pointers, sizes and ints; no
+ * HIP or torch types.  Device pointers are HIP device addresses; streams are
+ * passed as `void*` (a hipStream_t).  Errors are returned as status codes,
+ * never by abort(); bnsamd_last_error() gives the message of the context's
+ * last failure.
+ *
+ * Data layout (SoA, one double per point per field, field f of point i at
+ * base[f * ld + i], ld >= npts):
+ *   point block  21 fields  BNSAMD_PT_*   (MyEOSParams + M1Quantities)
+ *   distribution 28 fields  BNSAMD_DI_*   (NuDistributionParams), optional
+ *   output       20 fields  BNSAMD_OUT_*  (M1Opacities)
+ * Units are the reference's: nb [nm^-3], T and chemical potentials [MeV]
+ * (rest masses included), n [nm^-3], J [MeV nm^-3]; outputs eta_0
+ * [nm^-3 s^-1], eta [MeV nm^-3 s^-1], kappa [nm^-1].
+ */
+#ifndef BNS_NURATES_AMD_H_
+#define BNS_NURATES_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+  BNSAMD_OK = 0,
+  BNSAMD_ERR_INVALID = 1,       /* bad argument: null pointer, size, order */
+  BNSAMD_ERR_HIP = 2,           /* HIP runtime failure */
+  BNSAMD_ERR_NO_DEVICE = 3,     /* no usable HIP device */
+  BNSAMD_ERR_NO_QUADRATURE = 4, /* bnsamd_set_quadrature not called */
+  BNSAMD_ERR_NOMEM = 5          /* device allocation failed */
+};
+
+/* ---- field indices --------------------------------------------------------
+ * point block: MyEOSParams fields read by the path (bns_nurates.hpp:289-306)
+ * then M1Quantities n, J, chi per species (bns_nurates.hpp:409-416); species
+ * order nue, anue, nux, anux (bns_nurates.hpp:57-60). */
+enum {
+  BNSAMD_PT_NB = 0, BNSAMD_PT_TEMP = 1, BNSAMD_PT_YP = 2, BNSAMD_PT_YN = 3,
+  BNSAMD_PT_MU_P = 4, BNSAMD_PT_MU_N = 5, BNSAMD_PT_MU_E = 6, BNSAMD_PT_DU = 7,
+  BNSAMD_PT_DM_EFF = 8, BNSAMD_PT_N = 9, BNSAMD_PT_J = 13, BNSAMD_PT_CHI = 17,
+  BNSAMD_PT_NFIELDS = 21
+};
+/* distribution block: NuDistributionParams (bns_nurates.hpp:387-401) */
+enum {
+  BNSAMD_DI_W_T = 0, BNSAMD_DI_TEMP_T = 4, BNSAMD_DI_ETA_T = 8, BNSAMD_DI_W_F = 12,
+  BNSAMD_DI_TEMP_F = 16, BNSAMD_DI_C_F = 20, BNSAMD_DI_BETA_F = 24,
+  BNSAMD_DI_NFIELDS = 28
+};
+/* output block: M1Opacities (bns_nurates.hpp:463-474) */
+enum {
+  BNSAMD_OUT_ETA_0 = 0, BNSAMD_OUT_KAPPA_0_A = 4, BNSAMD_OUT_ETA = 8,
+  BNSAMD_OUT_KAPPA_A = 12, BNSAMD_OUT_KAPPA_S = 16, BNSAMD_OUT_NFIELDS = 20
+};
+
+/* OpacityFlags (bns_nurates.hpp:418-426), same field order and types */
+typedef struct bnsamd_flags {
+  int use_abs_em, use_pair, use_brem, use_inelastic_scatt, use_iso;
+} bnsamd_flags;
+
+/* OpacityParams (bns_nurates.hpp:338-352) with every bool as int;
+ * brem_implementation: 0 BREM_HR98, 1 BREM_BRT06, 2 BREM_GP19 (adds zero, as
+ * the reference's table-free configuration does, kernel_brem_GP19.hpp:483) */
+typedef struct bnsamd_params {
+  int use_dU, use_dm_eff, use_WM_ab, use_WM_sc, use_decay, brem_implementation,
+      use_NN_medium_corr, neglect_blocking;
+} bnsamd_params;
+
+/* precision of a gray opacity evaluation */
+enum { BNSAMD_FP64 = 0, BNSAMD_MIXED = 1 };
+
+typedef struct bnsamd_context bnsamd_context;
+
+/* ---- context --------------------------------------------------------------
+ * One context per device; it owns a stream (used until bnsamd_set_stream
+ * selects another; NULL selects HIP's default stream), the resident
+ * quadrature tables and the staging buffers of the host entry point.  Calls on one context are serialised by
+ * an internal lock; results never depend on how points are batched. */
+int bnsamd_create(int device, bnsamd_context** out);
+void bnsamd_destroy(bnsamd_context* ctx);
+const char* bnsamd_last_error(const bnsamd_context* ctx);
+int bnsamd_set_stream(bnsamd_context* ctx, void* hip_stream);
+void* bnsamd_get_stream(bnsamd_context* ctx);
+int bnsamd_synchronize(bnsamd_context* ctx);
+int bnsamd_version(void);
+
+/* GaussLegendre on [x1, x2] (integration.hpp:28-87): the same Newton
+ * iteration and stopping rule, host code, any n >= 1. */
+int bnsamd_gauss_legendre(int n, double x1, double x2, double* points, double* weights);
+
+/* Builds (or re-selects from the context's cache) the resident tables for a
+ * 1D quadrature (n1 nodes) and a 2D quadrature (n2 nodes) on [0, 1], as
+ * ComputeM1Opacities receives quad_1d and quad_2d
+ * (m1_opacities.hpp:1306-1313).  1 <= n1, n2 <= 32; the reference caps 2n at
+ * BS_N_MAX = 20 (bns_nurates.hpp:68). */
+int bnsamd_set_quadrature(bnsamd_context* ctx, int n1, const double* x1, const double* w1,
+                          int n2, const double* x2, const double* w2);
+
+/* Gray M1 opacities of npts points, device buffers, asynchronous on the
+ * context stream.  Replaces ComputeM1Opacities (stim_abs = 1) and
+ * ComputeM1OpacitiesNotStimulated (stim_abs = 0), m1_opacities.hpp:1297-1313,
+ * called once per point by the reference's drivers (tests/tests.hpp:756).
+ * d_distr may be NULL: the distribution parameters are then reconstructed on
+ * the device from (n, J, chi) as CalculateDistrParamsFromM1 does
+ * (distribution.hpp:323-333, the callers' step at tests/tests.hpp:731).
+ * precision: BNSAMD_FP64, or BNSAMD_MIXED (FP32 kernels and integrands, FP64
+ * Fermi integrals, detailed balance and final sums). */
+int bnsamd_m1_opacities(bnsamd_context* ctx, int64_t npts, int64_t ld, const double* d_pts,
+                        const double* d_distr, const bnsamd_flags* flags,
+                        const bnsamd_params* params, int stim_abs, int precision,
+                        double* d_out);
+
+/* Same, from and to host memory (synchronous; stages through the context's
+ * device buffers).  ld of the host blocks is npts. */
+int bnsamd_m1_opacities_host(bnsamd_context* ctx, int64_t npts, const double* h_pts,
+                             const double* h_distr, const bnsamd_flags* flags,
+                             const bnsamd_params* params, int stim_abs, int precision,
+                             double* h_out);
+
+/* CalculateDistrParamsFromM1 (distribution.hpp:323-333) on the device:
+ * d_distr[28][ld] from the n, J, chi fields of d_pts. */
+int bnsamd_distr_from_m1(bnsamd_context* ctx, int64_t npts, int64_t ld, const double* d_pts,
+                         double* d_distr);
+
+/* NuEquilibriumParams + ComputeM1DensitiesEq (distribution.hpp:253-290,
+ * 447-473) on the device: writes the equilibrium distribution into d_distr
+ * and the equilibrium n, J into the point block d_pts (chi untouched). */
+int bnsamd_equilibrium(bnsamd_context* ctx, int64_t npts, int64_t ld, double* d_pts,
+                       double* d_distr);
+
+/* Complete Fermi-Dirac integral F_k, k = 0..5 (FDI_0..FDI_p5,
+ * functions.hpp:1551-2837), elementwise on device arrays. */
+int bnsamd_fdi(bnsamd_context* ctx, int k, int64_t n, const double* d_y, double* d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BNS_NURATES_AMD_H_ */

@@ -5,6 +5,8 @@ the cpu_baseline leg of bench.py -- as the checker, never as the product.
 
 * ``liborc.so``      -- the CPU restatement of the gray M1 path (FP64)
 * ``liborc_count.so`` -- the same code over a FLOP-counting real type
+* ``liborc_bound.so`` -- the same code over a magnitude-propagating type: a
+  running error bound m per output (see oracle/error_bound.cpp)
 * ``_ref/libbnsref.so`` -- the genuine reference compiled from its headers
   (n <= 10 only: BS_N_MAX = 20 in bns_nurates.hpp:68)
 
@@ -22,6 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBS = {
     "orc": os.path.join(HERE, "liborc.so"),
     "count": os.path.join(HERE, "liborc_count.so"),
+    "bound": os.path.join(HERE, "liborc_bound.so"),
     "ref": os.path.join(HERE, "_ref", "libbnsref.so"),
 }
 _cache = {}
@@ -51,6 +54,13 @@ def load(kind: str = "orc"):
         raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
     lib = C.CDLL(path)
     p = "ref" if kind == "ref" else "orc"
+    if kind == "bound":
+        f = lib.orc_bound_m1_opacities
+        f.argtypes = [_i64, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_void_p, _ip, _ip,
+                      C.c_int, _dp, _dp, C.c_int]
+        f.restype = C.c_int
+        _cache[kind] = (lib, p)
+        return _cache[kind]
     if kind == "count":
         f = lib.orc_count_m1_opacities
         f.argtypes = [_i64, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, _ip, _ip, C.c_int, _dp, _i64p]
@@ -212,3 +222,57 @@ def count_m1_opacities(pts, nquad, flags, params, stim=1):
                                np.ascontiguousarray(params, np.int32), int(stim), out, counts)
     names = ["add", "mul", "div", "sqrt", "exp", "log", "pow", "atan", "cbrt"]
     return dict(zip(names, (int(c) for c in counts))), out
+
+
+U = 2.0 ** -53  # unit roundoff of IEEE double
+
+
+def bound_m1_opacities(pts, nquad, flags, params, stim=1, distr=None, nthreads=0,
+                       nquad_1d=None):
+    """Outputs [20, npts] and their running error magnitudes m [20, npts]:
+    any double evaluation of the path lies within ~k U m of the exact value."""
+    lib, _ = load("bound")
+    pts = _f64(pts)
+    npts = pts.shape[1]
+    n1 = nquad if nquad_1d is None else nquad_1d
+    x1, w1 = gauss_legendre(n1)
+    x2, w2 = gauss_legendre(nquad)
+    out = np.zeros((20, npts))
+    mag = np.zeros((20, npts))
+    dptr = None
+    if distr is not None:
+        distr = _f64(distr)
+        dptr = distr.ctypes.data_as(C.c_void_p)
+    lib.orc_bound_m1_opacities(npts, n1, x1, w1, nquad, x2, w2, pts, dptr,
+                               np.ascontiguousarray(flags, np.int32),
+                               np.ascontiguousarray(params, np.int32), int(stim), out, mag,
+                               int(nthreads))
+    return out, mag
+
+
+def bound_distr_from_m1(pts):
+    """Distribution parameters [28, npts] reconstructed from M1 and their
+    running error magnitudes m [28, npts]."""
+    lib, _ = load("bound")
+    pts = _f64(pts)
+    npts = pts.shape[1]
+    d = np.zeros((28, npts))
+    mag = np.zeros((28, npts))
+    f = lib.orc_bound_distr_from_m1
+    f.argtypes = [_i64, _dp, _dp, _dp]
+    f.restype = None
+    f(npts, pts, d, mag)
+    return d, mag
+
+
+def parity_excess(got, ref, mag, rtol=1e-12, k=64.0):
+    """Elementwise |got - ref| / (rtol |ref| + k U m): <= 1 passes the
+    condition-aware parity bar.  Equal values (incl. both NaN) give 0."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    same = (got == ref) | (np.isnan(got) & np.isnan(ref))
+    tol = rtol * np.abs(ref) + k * U * np.abs(mag) + 1e-300
+    with np.errstate(invalid="ignore"):
+        ex = np.abs(got - ref) / tol
+    ex = np.where(same, 0.0, ex)
+    return np.where(np.isnan(ex), np.inf, ex)
