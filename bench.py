@@ -1,4 +1,4 @@
-"""Throughput of the MI355X gray M1 opacity path on the config-1 workload.
+"""Throughput of the MI355X gray M1 opacity path on the 128^3-point grid workload.
 
 One step = one pass of ComputeM1Opacities (all reactions, BRT06
 bremsstrahlung, Gauss-Legendre n = 16 -> 32 energy nodes) over the 128^3 =
@@ -29,13 +29,24 @@ METRIC = "M1-opacity points/s, FP64 plain vs mixed/surrogate at stated rel err; 
 
 # Algorithmic work per point of the gray path at n = 16, all reactions, from
 # the FLOP-counting build of the CPU restatement (oracle/flop_count.cpp) on the
-# config-1 distributions (+, -, *, /, sqrt = 1 FLOP each; exp/log/pow/atan/cbrt
+# 128^3-grid workload (+, -, *, /, sqrt = 1 FLOP each; exp/log/pow/atan/cbrt
 # tallied apart).  tests/test_bench_constants.py re-derives them.
 FLOPS_PER_POINT = {"brt06": 1114336.0, "hr98": 1361608.0}
 TRANSCENDENTALS_PER_POINT = {"brt06": {"exp": 19309.3, "pow": 5528.0, "log": 197.7},
                              "hr98": {"exp": 23201.3, "pow": 38264.0, "log": 1.7,
                                       "atan": 3168.0, "cbrt": 528.0}}
 PEAK_FP64_VECTOR_TFLOPS = 78.6  # MI355X datasheet, FP64 vector (FMA = 2)
+
+# Measured with rocprofv3 on k_gray_m1_f64 at n = 16, BRT06
+# (profiles/k_gray_m1_f64_grid128_n16_brt06.md):
+# HBM bytes per point per launch, FETCH_SIZE (x2, calibrated on a 1 GiB copy)
+# + WRITE_SIZE (x1) over the 2,097,152-point launch ...
+TRAFFIC_BYTES_PER_POINT = (346528 + 327680) * 1024 / 2097152
+# ... against the algorithmic 21 input + 20 output doubles per point
+ALGORITHMIC_BYTES_PER_POINT = 8 * (21 + 20)
+# hardware FP64 FLOPs per point, (ADD + MUL + TRANS + 2 FMA) x 64 lanes from
+# the SQ_INSTS_VALU_*_F64 counters (262,144 points)
+HW_FLOPS_PER_POINT = 2118002.8
 
 
 def log(*a):
@@ -80,7 +91,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(points shard with no exchange, see DESIGN.md)", file=sys.stderr)
+              "(points shard across GPUs with no exchange; only the one-GPU path is built)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -90,7 +101,7 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
-    log(f"generating {args.npts} points (config-1 distributions, seed 1)")
+    log(f"generating {args.npts} points (128^3-grid workload distributions, seed 1)")
     pts = W.make_points(args.npts, seed=1)
     dpts = torch.from_numpy(pts).cuda()
     out = torch.empty((W.O_NFIELDS, args.npts), dtype=torch.float64, device="cuda")
@@ -150,16 +161,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "cfg1_grid128_n16_brt06", "npts": args.npts,
+        "config": {"workload": "grid128_n16_brt06", "npts": args.npts,
                    "nquad": args.nquad, "energy_nodes": 2 * args.nquad,
                    "reactions": "beta+pair+brem+nes+iso", "brem": "BRT06",
                    "opacity_params": "default_all", "distribution": "reconstructed from M1"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VECTOR_TFLOPS,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_VECTOR_TFLOPS,
                      "unit": "TFLOP/s",
                      "frac": (achieved / PEAK_FP64_VECTOR_TFLOPS) if achieved else None,
-                     "traffic": None,
+                     "traffic": TRAFFIC_BYTES_PER_POINT * args.npts if args.nquad == 16 else None,
+                     "bound_detail": "FP64 compute on the vector ALU (MI355X FP64 vector and "
+                                     "matrix peaks are both 78.6 TFLOP/s); HBM traffic "
+                                     f"{TRAFFIC_BYTES_PER_POINT:.1f} B/point vs algorithmic "
+                                     f"{ALGORITHMIC_BYTES_PER_POINT} B/point",
                      "kernel": "k_gray_m1_f64", "kernel_ms": b["kernel_ms"],
                      "flops_per_point": FLOPS_PER_POINT["brt06"],
+                     "hw_flops_per_point": HW_FLOPS_PER_POINT,
+                     "hw_achieved": HW_FLOPS_PER_POINT * args.npts / (b["kernel_ms"] * 1e-3) / 1e12,
                      "transcendentals_per_point": TRANSCENDENTALS_PER_POINT["brt06"]},
     }
     if "hr98" in res:

@@ -52,6 +52,30 @@ int hip_fail(bnsamd_context* c, hipError_t e, const char* what) {
   return fail(c, BNSAMD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// grows the context's staging buffer of the host entry points to n doubles
+int ensure_stage(bnsamd_context* c, size_t n, const char* who) {
+  if (n <= c->stage_doubles) return BNSAMD_OK;
+  if (c->stage) (void)hipFree(c->stage);
+  c->stage = nullptr;
+  c->stage_doubles = 0;
+  if (hipMalloc(&c->stage, n * sizeof(double)) != hipSuccess)
+    return fail(c, BNSAMD_ERR_NOMEM, std::string(who) + ": hipMalloc");
+  c->stage_doubles = n;
+  return BNSAMD_OK;
+}
+
+int upload(bnsamd_context* c, double* d, const double* h, size_t n) {
+  return hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream) ==
+                 hipSuccess ? BNSAMD_OK : BNSAMD_ERR_HIP;
+}
+
+// copies n doubles back and waits for the stream
+int download(bnsamd_context* c, double* h, const double* d, size_t n, const char* who) {
+  hipError_t e = hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? BNSAMD_OK : hip_fail(c, e, who);
+}
+
 double gamma_stirling_36() {
   // GammaStirling (functions.hpp:4460-4470) at c_f + 3 = 3.6
   const double x = 0.6 + 3.0;
@@ -86,6 +110,22 @@ __global__ void k_equilibrium(int64_t n, int64_t ld, double* __restrict__ pts,
     distr[(BNSAMD_DI_BETA_F + s) * ld + i] = 1.;
     double F[6];
     bns::fdi_bundle<2, 3>(eta[s], F);
+    pts[(BNSAMD_PT_N + s) * ld + i] = npre * F[2];
+    pts[(BNSAMD_PT_J + s) * ld + i] = jpre * F[3];
+  }
+}
+
+// ComputeM1DensitiesEq (distribution.hpp:447-473) for a given distribution:
+// n, J from eta_t and T, one point per thread
+__global__ void k_m1_densities_eq(int64_t n, int64_t ld, const double* __restrict__ distr,
+                                  double* __restrict__ pts) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double T = pts[BNSAMD_PT_TEMP * ld + i];
+  double npre = bns::c::FourPiHc3 * (T * T * T), jpre = npre * T;
+  for (int s = 0; s < 4; ++s) {
+    double F[6];
+    bns::fdi_bundle<2, 3>(distr[(BNSAMD_DI_ETA_T + s) * ld + i], F);
     pts[(BNSAMD_PT_N + s) * ld + i] = npre * F[2];
     pts[(BNSAMD_PT_J + s) * ld + i] = jpre * F[3];
   }
@@ -332,14 +372,7 @@ int bnsamd_m1_opacities_host(bnsamd_context* c, int64_t npts, const double* h_pt
   (void)hipSetDevice(c->device);
   size_t need = size_t(npts) * (BNSAMD_PT_NFIELDS + BNSAMD_OUT_NFIELDS +
                                 (h_distr ? BNSAMD_DI_NFIELDS : 0));
-  if (need > c->stage_doubles) {
-    if (c->stage) (void)hipFree(c->stage);
-    c->stage = nullptr;
-    c->stage_doubles = 0;
-    if (hipMalloc(&c->stage, need * sizeof(double)) != hipSuccess)
-      return fail(c, BNSAMD_ERR_NOMEM, "bnsamd_m1_opacities_host: hipMalloc");
-    c->stage_doubles = need;
-  }
+  if (int rc = ensure_stage(c, need, "bnsamd_m1_opacities_host")) return rc;
   double* dp = c->stage;
   double* dout = dp + size_t(npts) * BNSAMD_PT_NFIELDS;
   double* dd = h_distr ? dout + size_t(npts) * BNSAMD_OUT_NFIELDS : nullptr;
@@ -386,6 +419,71 @@ int bnsamd_equilibrium(bnsamd_context* c, int64_t npts, int64_t ld, double* d_pt
                      d_distr);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? BNSAMD_OK : hip_fail(c, e, "k_equilibrium");
+}
+
+int bnsamd_m1_densities_eq(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_distr,
+                           double* d_pts) {
+  if (check_ctx(c)) return BNSAMD_ERR_INVALID;
+  if (npts < 0 || ld < npts || !d_pts || !d_distr)
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_m1_densities_eq: invalid argument");
+  if (npts == 0) return BNSAMD_OK;
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  unsigned blocks = unsigned((npts + 255) / 256);
+  hipLaunchKernelGGL(k_m1_densities_eq, dim3(blocks), dim3(256), 0, c->stream, npts, ld,
+                     d_distr, d_pts);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? BNSAMD_OK : hip_fail(c, e, "k_m1_densities_eq");
+}
+
+// ---- host-memory forms of the distribution set-up (synchronous) -------------
+// op 0: distr_from_m1 (pts -> distr), 1: equilibrium (pts -> distr, pts n/J),
+// 2: m1_densities_eq (distr -> pts n/J)
+static int distr_host(bnsamd_context* c, int op, int64_t npts, double* h_pts, double* h_distr,
+                      const char* who) {
+  if (check_ctx(c)) return BNSAMD_ERR_INVALID;
+  if (npts < 0 || !h_pts || !h_distr) return fail(c, BNSAMD_ERR_INVALID, std::string(who) +
+                                                  ": invalid argument");
+  if (npts == 0) return BNSAMD_OK;
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  const size_t np = size_t(npts), P = np * BNSAMD_PT_NFIELDS, D = np * BNSAMD_DI_NFIELDS;
+  if (int rc = ensure_stage(c, P + D, who)) return rc;
+  double* dp = c->stage;
+  double* dd = dp + P;
+  if (upload(c, dp, h_pts, P) || (op == 2 && upload(c, dd, h_distr, D)))
+    return fail(c, BNSAMD_ERR_HIP, std::string(who) + ": upload");
+  unsigned blocks = unsigned((npts + 255) / 256);
+  if (op == 0)
+    hipLaunchKernelGGL(k_distr_from_m1, dim3(blocks), dim3(256), 0, c->stream, npts, npts, dp,
+                       dd, gamma_stirling_36());
+  else if (op == 1)
+    hipLaunchKernelGGL(k_equilibrium, dim3(blocks), dim3(256), 0, c->stream, npts, npts, dp, dd);
+  else
+    hipLaunchKernelGGL(k_m1_densities_eq, dim3(blocks), dim3(256), 0, c->stream, npts, npts, dd,
+                       dp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(c, e, who);
+  if (op != 2)
+    if (int rc = download(c, h_distr, dd, D, who)) return rc;
+  if (op != 0)
+    if (int rc = download(c, h_pts, dp, P, who)) return rc;
+  return BNSAMD_OK;
+}
+
+int bnsamd_distr_from_m1_host(bnsamd_context* c, int64_t npts, const double* h_pts,
+                              double* h_distr) {
+  return distr_host(c, 0, npts, const_cast<double*>(h_pts), h_distr, "bnsamd_distr_from_m1_host");
+}
+
+int bnsamd_equilibrium_host(bnsamd_context* c, int64_t npts, double* h_pts, double* h_distr) {
+  return distr_host(c, 1, npts, h_pts, h_distr, "bnsamd_equilibrium_host");
+}
+
+int bnsamd_m1_densities_eq_host(bnsamd_context* c, int64_t npts, const double* h_distr,
+                                double* h_pts) {
+  return distr_host(c, 2, npts, h_pts, const_cast<double*>(h_distr),
+                    "bnsamd_m1_densities_eq_host");
 }
 
 int bnsamd_fdi(bnsamd_context* c, int k, int64_t n, const double* d_y, double* d_out) {

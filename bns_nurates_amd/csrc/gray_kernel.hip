@@ -25,6 +25,17 @@ namespace {
 
 constexpr int kWave = 64;
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Give XCD x the contiguous range of points
+// [x q + min(x, r), ...) of length q + (x < r), q = G / 8, r = G % 8, so that
+// the 8 points sharing a 64-byte line of a field row are read and written
+// through one L2.  A bijection of [0, G) for any grid size G.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_point(int64_t b, int64_t G) {
+  int64_t x = b % kXcds, slot = b / kXcds, q = G / kXcds, r = G % kXcds;
+  return x * q + (x < r ? x : r) + slot;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -164,7 +175,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
   __shared__ HrChan s_hc[3];
 
   const int lane = threadIdx.x;
-  const int64_t p = blockIdx.x;
+  const int64_t p = xcd_point(blockIdx.x, gridDim.x);
   if (p >= A.npts) return;
   const int64_t ld = A.ld;
   const double* __restrict__ tb = A.tab;
@@ -811,8 +822,10 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
       }
     }
 #pragma unroll
-    for (int i = 0; i < 20; ++i) A.out[i * ld + p] = o[i];
+    for (int i = 0; i < 20; ++i) s_in[i] = o[i];
   }
+  __syncthreads();
+  if (lane < 20) A.out[lane * ld + p] = vin[lane];
 }
 
 }  // namespace bns

@@ -190,7 +190,7 @@ def make_points(npts: int, seed: int = 0, rho_range=(1e6, 1e15), T_range=(0.5, 5
 
 # The named workloads (sizes, quadrature order, reactions, bremsstrahlung)
 WORKLOADS = {
-    "cfg0_n10_brt06": dict(npts=4096, nquad=10, brem=BREM_BRT06, seed=0),
-    "cfg1_grid128_n16_brt06": dict(npts=128 ** 3, nquad=16, brem=BREM_BRT06, seed=1),
-    "cfg1_grid128_n16_hr98": dict(npts=128 ** 3, nquad=16, brem=BREM_HR98, seed=1),
+    "pts4096_n10_brt06": dict(npts=4096, nquad=10, brem=BREM_BRT06, seed=0),
+    "grid128_n16_brt06": dict(npts=128 ** 3, nquad=16, brem=BREM_BRT06, seed=1),
+    "grid128_n16_hr98": dict(npts=128 ** 3, nquad=16, brem=BREM_HR98, seed=1),
 }

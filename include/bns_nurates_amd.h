@@ -133,6 +133,23 @@ int bnsamd_distr_from_m1(bnsamd_context* ctx, int64_t npts, int64_t ld, const do
 int bnsamd_equilibrium(bnsamd_context* ctx, int64_t npts, int64_t ld, double* d_pts,
                        double* d_distr);
 
+/* ComputeM1DensitiesEq (distribution.hpp:447-473) on the device: the n, J
+ * fields of d_pts from the eta_t fields of d_distr and the temperature. */
+int bnsamd_m1_densities_eq(bnsamd_context* ctx, int64_t npts, int64_t ld, const double* d_distr,
+                           double* d_pts);
+
+/* Host-memory forms of the three set-up steps above (synchronous, staged
+ * through the context's device buffers; ld = npts).  They stand in for the
+ * per-point host calls CalculateDistrParamsFromM1 (distribution.hpp:324),
+ * NuEquilibriumParams (:254) + ComputeM1DensitiesEq (:448) and
+ * ComputeM1DensitiesEq alone, which the reference's drivers make before
+ * ComputeM1Opacities (tests/tests.hpp:731, mwe.cpp:192-200,307). */
+int bnsamd_distr_from_m1_host(bnsamd_context* ctx, int64_t npts, const double* h_pts,
+                              double* h_distr);
+int bnsamd_equilibrium_host(bnsamd_context* ctx, int64_t npts, double* h_pts, double* h_distr);
+int bnsamd_m1_densities_eq_host(bnsamd_context* ctx, int64_t npts, const double* h_distr,
+                                double* h_pts);
+
 /* Complete Fermi-Dirac integral F_k, k = 0..5 (FDI_0..FDI_p5,
  * functions.hpp:1551-2837), elementwise on device arrays. */
 int bnsamd_fdi(bnsamd_context* ctx, int k, int64_t n, const double* d_y, double* d_out);

@@ -142,15 +142,15 @@ def main():
          total_nuf=tnuf, total_nuf_points=np.arange(0, dp.shape[1], 8))
 
     # --- gray M1 opacities end to end ---------------------------------------
-    cfg = W.WORKLOADS["cfg0_n10_brt06"]
+    cfg = W.WORKLOADS["pts4096_n10_brt06"]
     p0 = W.make_points(cfg["npts"], seed=cfg["seed"])
     out = O.m1_opacities(p0, 10, W.FLAGS_ALL, W.params_default_all(W.BREM_BRT06), kind="ref")
-    save("m1_cfg0_n10_brt06.npz", pts=p0, out=out, nquad=10, flags=W.FLAGS_ALL,
+    save("m1_pts4096_n10_brt06.npz", pts=p0, out=out, nquad=10, flags=W.FLAGS_ALL,
          params=W.params_default_all(W.BREM_BRT06), stim=1)
 
     sub = np.ascontiguousarray(p0[:, :512])
     out = O.m1_opacities(sub, 10, W.FLAGS_ALL, W.params_default_all(W.BREM_HR98), kind="ref")
-    save("m1_cfg0_n10_hr98.npz", pts=sub, out=out, nquad=10, flags=W.FLAGS_ALL,
+    save("m1_pts512_n10_hr98.npz", pts=sub, out=out, nquad=10, flags=W.FLAGS_ALL,
          params=W.params_default_all(W.BREM_HR98), stim=1)
 
     ep = edge_points()

@@ -1,5 +1,5 @@
 """The algorithmic FLOP figures bench.py reports per point agree with the
-counting build of the oracle on the bench workload (config-1 distributions,
+counting build of the oracle on the bench workload (128^3-grid distributions,
 n = 16)."""
 import numpy as np
 import pytest

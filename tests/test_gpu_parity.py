@@ -77,23 +77,23 @@ def test_distribution_device_vs_reference(ctx, golden):
     assert r.max() <= 1e-13
 
 
-def test_cfg0_n10_brt06_vs_reference(ctx, golden):
-    g = golden("m1_cfg0_n10_brt06.npz")
-    say("config 0: 4096 points, n=10, BRT06 vs reference outputs")
+def test_pts4096_n10_brt06_vs_reference(ctx, golden):
+    g = golden("m1_pts4096_n10_brt06.npz")
+    say("4096 points, n=10, BRT06 vs reference outputs")
     out = run(ctx, g["pts"], 10, g["flags"], g["params"], int(g["stim"]))
     v, m = O.bound_m1_opacities(g["pts"], 10, g["flags"], g["params"], int(g["stim"]))
     # the host's oracle reproduces the reference (bit for bit where its libm
     # rounds as the one the fixtures were made with)
     assert O.parity_excess(v, g["out"], m).max() <= 1.0
-    assert check_parity("cfg0 brt06", out, g["out"], m, W.OUT_NAMES) <= 1.0
+    assert check_parity("4096 pts brt06", out, g["out"], m, W.OUT_NAMES) <= 1.0
 
 
-def test_cfg0_n10_hr98_vs_reference(ctx, golden):
-    g = golden("m1_cfg0_n10_hr98.npz")
-    say("config 0 subset: 512 points, n=10, HR98 vs reference outputs")
+def test_pts512_n10_hr98_vs_reference(ctx, golden):
+    g = golden("m1_pts512_n10_hr98.npz")
+    say("512-point subset, n=10, HR98 vs reference outputs")
     out = run(ctx, g["pts"], 10, g["flags"], g["params"], int(g["stim"]))
     _, m = O.bound_m1_opacities(g["pts"], 10, g["flags"], g["params"], int(g["stim"]))
-    assert check_parity("cfg0 hr98", out, g["out"], m, W.OUT_NAMES) <= 1.0
+    assert check_parity("512 pts hr98", out, g["out"], m, W.OUT_NAMES) <= 1.0
 
 
 def test_variants_vs_reference(ctx, golden):
@@ -118,7 +118,7 @@ def test_variants_vs_reference(ctx, golden):
 
 @pytest.mark.parametrize("brem", [W.BREM_BRT06, W.BREM_HR98])
 def test_n16_vs_oracle(ctx, brem):
-    """Config-1 distributions at n = 16 (beyond the reference's BS_N_MAX):
+    """The 128^3-grid distributions at n = 16 (beyond the reference's BS_N_MAX):
     against the n-generic oracle on a seeded sample."""
     pts = W.make_points(768, seed=1)
     prm = W.params_default_all(brem)
@@ -158,24 +158,24 @@ def test_host_entry_point_matches_device(ctx):
     assert np.array_equal(a, b)
 
 
-def test_cfg1_full_grid_properties(ctx):
-    """BASELINE config 1 at full size (128^3 points, n = 16): every output
+def test_grid128_full_properties(ctx):
+    """The 128^3-point grid at full size (n = 16): every output
     finite where the inputs are regular, kappa zero exactly where the
     thresholds say so, and a seeded sample equal to the oracle."""
-    cfg = W.WORKLOADS["cfg1_grid128_n16_brt06"]
-    say("config 1: generating 2,097,152 points")
+    cfg = W.WORKLOADS["grid128_n16_brt06"]
+    say("grid128: generating 2,097,152 points")
     pts = W.make_points(cfg["npts"], seed=cfg["seed"])
     prm = W.params_default_all(W.BREM_BRT06)
-    say("config 1: running the FP64 kernel")
+    say("grid128: running the FP64 kernel")
     out = run(ctx, pts, 16, W.FLAGS_ALL, prm)
     assert np.isfinite(out).all()
-    say("config 1: threshold zeros")
+    say("grid128: threshold zeros")
     n = pts[W.P_N0:W.P_N0 + 4]
     J = pts[W.P_J0:W.P_J0 + 4]
     assert (out[4:8][n <= 1e-21] == 0).all()
     assert (out[12:20][np.concatenate([J, J]) <= 1e-25] == 0).all()
     rng = np.random.default_rng(3)
     sel = np.sort(rng.choice(cfg["npts"], 384, replace=False))
-    say("config 1: oracle on a 384-point sample")
+    say("grid128: oracle on a 384-point sample")
     ref, m = O.bound_m1_opacities(np.ascontiguousarray(pts[:, sel]), 16, W.FLAGS_ALL, prm)
-    assert check_parity("cfg1 sample", out[:, sel], ref, m, W.OUT_NAMES) <= 1.0
+    assert check_parity("grid128 sample", out[:, sel], ref, m, W.OUT_NAMES) <= 1.0

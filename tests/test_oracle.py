@@ -261,14 +261,14 @@ def test_distribution_moments_roundtrip():
 # --------------------------------------------------------------------------
 # Gray M1 opacities end to end
 # --------------------------------------------------------------------------
-def test_m1_cfg0_matches_reference(golden):
-    g = golden("m1_cfg0_n10_brt06.npz")
+def test_m1_pts4096_matches_reference(golden):
+    g = golden("m1_pts4096_n10_brt06.npz")
     out = O.m1_opacities(g["pts"], int(g["nquad"]), g["flags"], g["params"], stim=int(g["stim"]))
     assert same(out, g["out"])
 
 
 def test_m1_hr98_matches_reference(golden):
-    g = golden("m1_cfg0_n10_hr98.npz")
+    g = golden("m1_pts512_n10_hr98.npz")
     out = O.m1_opacities(g["pts"], int(g["nquad"]), g["flags"], g["params"], stim=int(g["stim"]))
     assert same(out, g["out"])
 
