@@ -49,6 +49,34 @@ ALGORITHMIC_BYTES_PER_POINT = 8 * (21 + 20)
 HW_FLOPS_PER_POINT = 2118002.8
 
 
+# outputs whose accuracy the reduced-precision form reports: number and
+# energy emissivities, absorption and scattering coefficients of nue, anue, nux
+MIXED_ROWS = [0, 1, 2, 8, 9, 10, 4, 5, 6, 12, 13, 14, 16, 17, 18]
+MIXED_RTOL = 1e-5
+
+
+def mixed_error_stats(got, ref, mag):
+    """Accuracy of a reduced-precision result against the FP64 oracle.
+
+    Per output: [max, median] relative error over the points whose FP64
+    value is itself determined to 1e-7 (64 u m <= 1e-7 |ref|, m the running
+    error magnitude) and the fraction of such points; over every point, the
+    worst excess over the bar |got - ref| <= 1e-5 |ref| + 64 u m (<= 1
+    passes)."""
+    import oracle as O
+    from bns_nurates_amd import workloads as W
+    ex = O.parity_excess(got, ref, mag, rtol=MIXED_RTOL)
+    rel, det_frac = {}, {}
+    for r in MIXED_ROWS:
+        g, f, m = got[r], ref[r], mag[r]
+        det = (64 * O.U * np.abs(m) <= 1e-7 * np.abs(f)) & (f != 0)
+        e = np.abs(g[det] - f[det]) / np.abs(f[det]) if det.any() else np.zeros(1)
+        rel[W.OUT_NAMES[r]] = [float(f"{e.max():.3g}"), float(f"{np.median(e):.3g}")]
+        det_frac[W.OUT_NAMES[r]] = float(f"{det.mean():.3g}")
+    return {"bar_excess_max": float(f"{ex[MIXED_ROWS].max():.3g}"), "rel_err_max_median": rel,
+            "fp64_determined_frac": det_frac}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -84,7 +112,8 @@ def main():
     ap.add_argument("--npts", type=int, default=128 ** 3)
     ap.add_argument("--nquad", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-hr98", action="store_true", help="skip the HR98 pass")
+    ap.add_argument("--no-hr98", action="store_true", help="skip the HR98 passes")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision passes")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -118,15 +147,24 @@ def main():
         log(f"copy calibration: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
         del src, dst
 
-    def run_pass(params):
-        ctx.compute_m1_opacities(dpts, flags=W.FLAGS_ALL, params=params, out=out)
+    def run_pass(params, precision=api.FP64):
+        ctx.compute_m1_opacities(dpts, flags=W.FLAGS_ALL, params=params, out=out,
+                                 precision=precision)
 
+    # the kernel runs on torch's current stream, where the events are recorded
     res = {}
-    for brem in (["brt06"] if args.no_hr98 else ["brt06", "hr98"]):
-        params = W.params_default_all(W.BREM_BRT06 if brem == "brt06" else W.BREM_HR98)
+    passes = [("brt06", api.FP64)]
+    if not args.no_hr98:
+        passes.append(("hr98", api.FP64))
+    if not args.no_mixed:
+        passes.append(("brt06_mixed", api.MIXED))
+        if not args.no_hr98:
+            passes.append(("hr98_mixed", api.MIXED))
+    for brem, prec in passes:
+        params = W.params_default_all(W.BREM_BRT06 if brem.startswith("brt06") else W.BREM_HR98)
         steps = args.steps if brem == "brt06" else max(1, args.steps // 2)
         for _ in range(args.warmup):
-            run_pass(params)
+            run_pass(params, prec)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(steps)]
@@ -135,7 +173,7 @@ def main():
         t0 = time.perf_counter()
         for a, b in ev:
             a.record()
-            run_pass(params)
+            run_pass(params, prec)
             b.record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -184,6 +222,31 @@ def main():
         line["hr98"] = {"value": h["pps"], "unit": "points/s", "kernel_ms": h["kernel_ms"],
                         "achieved_tflops": FLOPS_PER_POINT["hr98"] * args.npts /
                         (h["kernel_ms"] * 1e-3) / 1e12 if args.nquad == 16 else None}
+    for brem in ("brt06", "hr98"):
+        mk = brem + "_mixed"
+        if mk not in res:
+            continue
+        h = res[mk]
+        params = W.params_default_all(W.BREM_BRT06 if brem == "brt06" else W.BREM_HR98)
+        # accuracy against the FP64 oracle: the first 2048 points of the
+        # workload and 512 cold degenerate tail points (T < 1 MeV, eta_e > 50)
+        import oracle as O
+        if not O.available("bound"):
+            O.build()
+        acc = {}
+        for tag, sp in (("sample", np.ascontiguousarray(pts[:, :2048])),
+                        ("tail_T<1_eta_e>50", W.make_tail_points(512, seed=5))):
+            ctx.set_quadrature(args.nquad)
+            got = ctx.compute_m1_opacities(torch.from_numpy(sp).cuda(), flags=W.FLAGS_ALL,
+                                           params=params, precision=api.MIXED)
+            torch.cuda.synchronize()
+            ref, mag = O.bound_m1_opacities(sp, args.nquad, W.FLAGS_ALL, params)
+            acc[tag] = mixed_error_stats(got.cpu().numpy(), ref, mag)
+        line.setdefault("mixed", {})[brem] = {
+            "value": h["pps"], "unit": "points/s", "kernel_ms": h["kernel_ms"],
+            "speedup_vs_fp64": res[brem]["kernel_ms"] / h["kernel_ms"] if brem in res else None,
+            "fp32_parts": "distribution values (exp/log), HR98 channel functions",
+            "rel_err_vs_fp64_oracle": acc}
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(pts, args.nquad, W.params_default_all(W.BREM_BRT06))

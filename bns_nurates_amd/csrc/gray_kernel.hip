@@ -47,12 +47,62 @@ __device__ __forceinline__ double wave_sum(double v) {
 //   f(w) = wt / (e^{(w - mut)/Tt} + 1) + wf bf w^cf e^{-w/Tf}
 struct Spec {
   double wt, mut, invTt, wf, bf, cf, invTf;
+  double lbf;  // log(bf), the mixed-precision form folds it into the exponent
 };
 
 __device__ __forceinline__ double nuf(double w, double lw, const Spec& d) {
   double arg = (w - d.mut) * d.invTt;
   double thin = d.bf * exp(fma(d.cf, lw, -w * d.invTf));
   return fma(d.wt, fermi_of_arg(arg), d.wf * thin);
+}
+
+// A distribution value f and 1 - f.
+struct GOmg {
+  double g, omg;
+};
+
+// e^x with FP32 accuracy over the whole FP64 range: x log2(e) = k + r with
+// k integral and |r| <= 1/2, 2^r by the hardware FP32 exponential, 2^k
+// applied exactly in FP64.  NaN maps to the lower clamp (e^x = 0), as
+// SafeExp does.
+__device__ __forceinline__ double exp_mixed(double x) {
+  x = fmin(fmax(x, -1100.), 709.);
+  const double u = x * 1.4426950408889634;
+  const double k = rint(u);
+  const float e2 = __builtin_amdgcn_exp2f(static_cast<float>(u - k));
+  return ldexp(static_cast<double>(e2), static_cast<int>(k));
+}
+
+// The mixed-precision TotalNuF: the exponent arguments are formed in FP64
+// (they can be hundreds, and w - mut cancels near the Fermi surface), the
+// exponentials, reciprocal and logarithm carry FP32 accuracy (relative error
+// ~1e-7 in f, no cancellation downstream) over the FP64 range, and 1 - f is
+// formed from 1 - FD = FD(-arg) without the cancellation of 1 - f at
+// degenerate energies.
+__device__ __forceinline__ GOmg nuf_mixed(double w, float lw, const Spec& d) {
+  const double arg = (w - d.mut) * d.invTt;
+  const double t = exp_mixed(fmax(-fabs(arg), -c::ExpLim));  // SafeExp's clamp
+  const double r = static_cast<double>(__builtin_amdgcn_rcpf(static_cast<float>(t + 1.)));
+  const double fd = (arg > 0. ? t : 1.) * r;  // 1 / (e^arg + 1)
+  const double fc = (arg > 0. ? 1. : t) * r;  // 1 - fd
+  const double thin = exp_mixed(fma(d.cf, static_cast<double>(lw), d.lbf - w * d.invTf));
+  GOmg o;
+  o.g = fma(d.wt, fd, d.wf * thin);
+  o.omg = fma(d.wt, fc, 1. - d.wt) - d.wf * thin;
+  return o;
+}
+
+// TotalNuF at w (and 1 - f) in the precision of the kernel variant
+template <bool MIX>
+__device__ __forceinline__ GOmg dist(double w, const Spec& d) {
+  GOmg o;
+  if constexpr (MIX) {
+    o = nuf_mixed(w, logf(static_cast<float>(w)), d);
+  } else {
+    o.g = nuf(w, log(w), d);
+    o.omg = 1. - o.g;
+  }
+  return o;
 }
 
 __device__ __forceinline__ Spec pick(const Spec* D, int s) {
@@ -149,7 +199,7 @@ struct HrChan {
   double weight;  // n_N * s-weights of the channel sum (incl. 28/3 for np)
   double gamma_half_gb;
   double gamma_over_T;
-  double sd_pt;   // s_d / (cell factor): 3 (pi/2)^2.5 es^-2.5 f_u
+  double lsd_pt;  // log of s_d / (cell factor): log(3 (pi/2)^2.5 es^-2.5 f_u)
   double f_pt;    // (1 + es^2)(1 + y^4)
   double g0;      // c1 y / (0.8 + 0.06 y^1.05) sqrt(es) / (es + 0.2)
   double h;       // h_brem
@@ -158,10 +208,11 @@ struct HrChan {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-template <int WAVES_PER_EU>
-__global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_f64(
-    GrayKernelArgs A) {
+// The body of both kernel variants: MIX = false is the FP64 path, MIX = true
+// the mixed-precision path (FP32 distribution values and HR98 channel
+// functions, everything else FP64).
+template <bool MIX>
+__device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
   __shared__ double s_Fp[5][64];  // F_1..F_5 (eta + y_a)
   __shared__ double s_g[4][64];   // TotalNuF(nu_a, s)
@@ -249,6 +300,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
       d.cf = 0.6;
       d.bf = nn_ / (c::FourPiHc3 * A.gamma36 * exp(3.6 * log(Tnu)));
     }
+    d.lbf = MIX ? log(d.bf) : 0.;
     s_D[s] = d;
   }
 
@@ -276,6 +328,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
     r.bf = vD[s].bf;
     r.cf = vD[s].cf;
     r.invTf = vD[s].invTf;
+    r.lbf = vD[s].lbf;
     return r;
   };
   const volatile double (*vFe)[6] = s_Fe;
@@ -304,7 +357,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
       double ts = s == 0 ? s_iso[0] : s == 1 ? s_iso[1] : s == 2 ? s_iso[2] : s_iso[3];
       double nu = half ? ts / x : ts * x;
       double W = half ? w / (x * x) : w;
-      double g = nuf(nu, log(nu), Dget(s));
+      double g = dist<MIX>(nu, Dget(s)).g;
       double isc = etaNNp * iso_leg(nu, 1, wm) + etaNNn * iso_leg(nu, 2, wm);
       double nu2 = nu * nu;
       double v = W * g * (4. * c::Pi) * nu2 * nu2 * nu * isc;
@@ -349,7 +402,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
       double ts = s == 0 ? s_beta[0] : s_beta[1];
       double nu = half ? ts / x : ts * x;
       double W = half ? w / (x * x) : w;
-      double g = nuf(nu, log(nu), Dget(s));
+      double g = dist<MIX>(nu, Dget(s)).g;
       double R = pp.use_WM_ab ? wm_abs(nu, s) : 1.;
       // AbsOpacitySingleLep (formulas_beta_iso.hpp:138-279) for species s:
       // captured lepton energy Ec (e- for nue, e+ for anue), decay lepton Ed
@@ -422,12 +475,19 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
         double xa = tb[L.nd + ND_REC * a + ND_X];
         double nu = tb[L.nd + ND_REC * a + ND_INV] != 0. ? tpair / xa : mul_rn(tpair, xa);
         s_y[a] = nu / T;
-        double lw = log(nu);
+        double lw = MIX ? 0. : log(nu);
+        float lw32 = MIX ? logf(static_cast<float>(nu)) : 0.f;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          double g = nuf(nu, lw, Dget(s));
-          s_g[s][a] = g;
-          s_bf[s][a] = blocking ? 1. - g : 1.;
+          GOmg go;
+          if constexpr (MIX) {
+            go = nuf_mixed(nu, lw32, Dget(s));
+          } else {
+            go.g = nuf(nu, lw, Dget(s));
+            go.omg = 1. - go.g;
+          }
+          s_g[s][a] = go.g;
+          s_bf[s][a] = blocking ? go.omg : 1.;
         }
       }
     }
@@ -495,7 +555,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
                      atan(1. / uarg) * uarg / 3.);
         fu = fabs(fu) < 1e-14 ? 1e-14 : fu;
         if (lane < 3) {
-          s_hc[ch].sd_pt = 3. * c::PiHalfToFiveHalves * exp(-2.5 * log(eg)) * fu;
+          s_hc[ch].lsd_pt = log(3. * c::PiHalfToFiveHalves * exp(-2.5 * log(eg)) * fu);
           s_hc[ch].f_pt = (1. + eg * eg) * (1. + y4);
           s_hc[ch].g0 = 0.0044 * ye / (0.8 + 0.06 * y105) * sqrt(eg) / (eg + 0.2);
           s_hc[ch].h = 0.1 * eg / (2.39 + 0.1 * exp(1.1 * log(eg)));
@@ -580,6 +640,41 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
         if (A.params.brem_impl == 1) {
           bem = brt_pre * cr[CL_K1EM];
           bab = brt_pre * cr[CL_K1AB];
+        } else if constexpr (MIX) {
+          // the same channel sum in FP32: smooth fits, no cancellation
+          float x = static_cast<float>(cr[CL_S]);
+          x = x > static_cast<float>(c::BremMin) ? x : static_cast<float>(c::BremMin);
+          float y = static_cast<float>(hr_y > c::BremMin ? hr_y : c::BremMin);
+          const float pf = static_cast<float>(hr_p), pinvf = static_cast<float>(hr_pinv);
+          float snum = static_cast<float>(2. * c::SqrtPi) *
+                       expf(1.5f * logf(x + 2.f - static_cast<float>(hr_emy))) *
+                       (x * x + 2.f * x * y + 5.f * (y * y) / 3.f + 1.f);
+          float bb = static_cast<float>(c::Pi2OneEighth) + x + y;
+          float b2 = bb * bb;
+          float snd = snum / (static_cast<float>(c::SqrtPi) + b2 * b2);
+          float lsdx = static_cast<float>(cr[CL_LSD]);
+          float pnd = expf(-pf * logf(snd));
+          float px11 = static_cast<float>(cr[CL_X11]), px093 = static_cast<float>(cr[CL_X093]);
+          float px12 = static_cast<float>(cr[CL_X12]), px28 = static_cast<float>(cr[CL_X28]);
+          float xm = x - 1.2f;
+          float fx = 3.f + xm * xm + static_cast<float>(cr[CL_XM4]);
+          float cbx = 30.f / (30.f + 0.005f * px28);
+          float tot = 0.f;
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            const volatile HrChan& h = s_hc[ch];
+            float hh = static_cast<float>(h.h), gx = static_cast<float>(h.gamma_half_gb);
+            float sd = expf(-pf * (static_cast<float>(h.lsd_pt) + lsdx));
+            float sbase = expf(pinvf * logf(pnd + sd));
+            float fb = 1.f + 1.f / (fx * static_cast<float>(h.f_pt));
+            float gbm = 1.f - px11 * static_cast<float>(h.g0);
+            float cb = 1.1f * px11 * hh / (2.3f + hh * px093 + 0.0001f * px12) * cbx;
+            float sb = sbase * fb * (1.f + cb * gbm);
+            float sabs = static_cast<float>(h.gamma_over_T) / (x * x + gx * gx) * sb;
+            tot += static_cast<float>(h.weight) * sabs;
+          }
+          bab = 3. * c::BremConst * static_cast<double>(tot) * hr_medium;
+          bem = bab * cr[CL_EMX];
         } else {
           // BremKernelS at x = (w+w')/T for the three channels
           double x = cr[CL_S];
@@ -600,7 +695,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
             const volatile HrChan& h = s_hc[ch];
-            double sd = exp(-hr_p * (log(h.sd_pt) + lsdx));
+            double sd = exp(-hr_p * (h.lsd_pt + lsdx));
             double sbase = exp(hr_pinv * log(pnd + sd));
             double fb = 1. + 1. / (fx * h.f_pt);
             double gbm = 1. - px11 * h.g0;
@@ -672,16 +767,6 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
       double xm = wp, ym = w;  // w < w' on the whole grid
       if (w > wp) { xm = w; ym = wp; }
       double sgn = hr[NP_SIGN];
-      double gn[4], gb[4];
-      {
-        double l1 = log(nu), l2 = log(nub);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          Spec d = Dget(s);
-          gn[s] = nuf(nu, l1, d);
-          gb[s] = nuf(nub, l2, d);
-        }
-      }
       double out[4] = {0., 0., 0., 0.};
 #pragma unroll
       for (int lep = 0; lep < 2; ++lep) {
@@ -706,14 +791,14 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
           double ws = w + wp;
           double inv = hr[NP_INVDEN];
           double q4 = fw3 + 2. * (ws * fw2 + 3. * w * wp * fw1);
+          // the out-kernel is linear in the couplings: b1 U + b2 V
+          double t1 = (sgn * fa2 - fw4) * 0.2;
+          double U = (t1 - ws * q4 + sgn * (xm * fa1 + 2. * (xm * xm) * fa0)) * inv - 6. * fw0;
+          double V = (t1 + sgn * (2. * (ym * ym) * fa0 - ym * fa1)) * inv;
+          U *= pre;
+          V *= pre;
 #pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            double t1 = (b1[s] + b2[s]) * (sgn * fa2 - fw4) * 0.2;
-            double t2 = b1[s] * ws * q4;
-            double t3 = sgn * ((b1[s] * xm - b2[s] * ym) * fa1 +
-                               2. * (b1[s] * xm * xm + b2[s] * ym * ym) * fa0);
-            out[s] = fma(pre, (t1 - t2 + t3) * inv - 6. * b1[s] * fw0, out[s]);
-          }
+          for (int s = 0; s < 4; ++s) out[s] = fma(b1[s], U, fma(b2[s], V, out[s]));
         } else if (xm > el * 1e-3) {
           // MezzacappaIntOneEnergy (kernel_nes.hpp:82-107)
           double D3[6];
@@ -743,6 +828,31 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
           }
         }
       }
+      // distribution values at nu and nubar, formed after the kernels so that
+      // they are not live through the Fermi-integral evaluations
+      double gn[4], gb[4], on[4], ob[4];  // f and 1 - f at nu and nubar
+      if constexpr (MIX) {
+        float l1 = logf(static_cast<float>(nu)), l2 = logf(static_cast<float>(nub));
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          Spec d = Dget(s);
+          GOmg a = nuf_mixed(nu, l1, d), b = nuf_mixed(nub, l2, d);
+          gn[s] = a.g;
+          on[s] = a.omg;
+          gb[s] = b.g;
+          ob[s] = b.omg;
+        }
+      } else {
+        double l1 = log(nu), l2 = log(nub);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          Spec d = Dget(s);
+          gn[s] = nuf(nu, l1, d);
+          gb[s] = nuf(nub, l2, d);
+          on[s] = 1. - gn[s];
+          ob[s] = 1. - gb[s];
+        }
+      }
       // in/out kernels for (nu, nubar) and the swapped pair: the swapped
       // kernel is the negated out-integral with NEPSExpFunc arguments
       // exchanged, i.e. em2 = ab1, ab2 = em1 (kernel_nes.hpp:229-234)
@@ -751,7 +861,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         double em1 = -out[s] * efx, ab1 = out[s] * ef;
-        double bfn = blocking ? 1. - gn[s] : 1., bfb = blocking ? 1. - gb[s] : 1.;
+        double bfn = blocking ? on[s] : 1., bfb = blocking ? ob[s] : 1.;
         double te1 = em1 * gb[s], ta1 = ab1 * bfb;
         double te2 = ab1 * gn[s], ta2 = em1 * bfn;
         double Ae, Aa, Be, Ba;
@@ -761,9 +871,9 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
           Be = te2;
           Ba = gb[s] * (te2 + ta2);
         } else {
-          Ae = (1. - gn[s]) * te1;
+          Ae = on[s] * te1;
           Aa = gn[s] * ta1;
-          Be = (1. - gb[s]) * te2;
+          Be = ob[s] * te2;
           Ba = gb[s] * ta2;
         }
         acc[s] = fma(wq, Ae + Be, acc[s]);
@@ -828,10 +938,24 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
   if (lane < 20) A.out[lane * ld + p] = vin[lane];
 }
 
+template <int WAVES_PER_EU>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_f64(
+    GrayKernelArgs A) {
+  gray_point<false>(A);
+}
+
+template <int WAVES_PER_EU>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_mixed(
+    GrayKernelArgs A) {
+  gray_point<true>(A);
+}
+
 }  // namespace bns
 
 #ifndef BNS_GRAY_WAVES_PER_EU
-#define BNS_GRAY_WAVES_PER_EU 2
+#define BNS_GRAY_WAVES_PER_EU 3
 #endif
 namespace bns {
 hipError_t launch_gray_m1_f64(const GrayKernelArgs& a, hipStream_t stream) {
@@ -843,5 +967,10 @@ hipError_t launch_gray_m1_f64(const GrayKernelArgs& a, hipStream_t stream) {
 }  // namespace bns
 
 namespace bns {
-hipError_t launch_gray_m1_mixed(const GrayKernelArgs&, hipStream_t) { return hipErrorNotSupported; }
+hipError_t launch_gray_m1_mixed(const GrayKernelArgs& a, hipStream_t stream) {
+  if (a.npts <= 0) return hipSuccess;
+  dim3 grid(static_cast<unsigned>(a.npts)), block(64);
+  hipLaunchKernelGGL(k_gray_m1_mixed<BNS_GRAY_WAVES_PER_EU>, grid, block, 0, stream, a);
+  return hipGetLastError();
+}
 }  // namespace bns

@@ -273,26 +273,39 @@ __device__ __forceinline__ double div_normal(double p, double q) {
 #endif
 }
 
+// Numerator and denominator of the two rational forms of F_k on x = -|y|:
+// x < -2: F = e^x (a0 - e^x p(t) / q(t)), t = e^{x+2};
+// -2 <= x <= 0: F = p(t) / q(s), s = -x/2, t = 1 - s.
 template <int K>
-BNS_HD double fdi_exp_branch(double ex, double t) {
+BNS_HD void fdi_exp_pq(double t, double& p, double& q) {
   using C = Fdi<K>;
-  double p = C::pa[0];
+  p = C::pa[0];
 #pragma unroll
   for (int i = 1; i < C::na; ++i) p = fma(t, p, C::pa[i]);
-  double q = t + C::qa[0];
+  q = t + C::qa[0];
 #pragma unroll
   for (int i = 1; i < C::nqa; ++i) q = fma(t, q, C::qa[i]);
-  return ex * (C::a0 - div_normal(ex * p, q));
+}
+template <int K>
+BNS_HD void fdi_rat_pq(double s, double t, double& p, double& q) {
+  using C = Fdi<K>;
+  p = C::pb[0];
+#pragma unroll
+  for (int i = 1; i < C::nb; ++i) p = fma(t, p, C::pb[i]);
+  q = C::qb_lead;
+#pragma unroll
+  for (int i = 0; i < C::nqb; ++i) q = fma(s, q, C::qb[i]);
+}
+template <int K>
+BNS_HD double fdi_exp_branch(double ex, double t) {
+  double p, q;
+  fdi_exp_pq<K>(t, p, q);
+  return ex * (Fdi<K>::a0 - div_normal(ex * p, q));
 }
 template <int K>
 BNS_HD double fdi_rat_branch(double s, double t) {
-  using C = Fdi<K>;
-  double p = C::pb[0];
-#pragma unroll
-  for (int i = 1; i < C::nb; ++i) p = fma(t, p, C::pb[i]);
-  double q = C::qb_lead;
-#pragma unroll
-  for (int i = 0; i < C::nqb; ++i) q = fma(s, q, C::qb[i]);
+  double p, q;
+  fdi_rat_pq<K>(s, t, p, q);
   return div_normal(p, q);
 }
 // exact polynomial added on reflection to y > 0 (functions.hpp:1606-2834)
@@ -335,10 +348,11 @@ struct FdiUnroll {
 // F_k(y) for k = KLO..KHI at one argument, sharing the branch decision and
 // the exponential (F[k] receives F_k; F must hold KHI + 1 entries).
 // Inlined: with MachineLICM disabled at build time the coefficient
-// materializations stay next to their use inside the cell loops.
-#define BNS_FDI_ATTR BNS_HD
+// materializations stay next to their use inside the cell loops.  (A
+// batched reciprocal of the five denominators saves divisions but keeps
+// more values live; at three waves per SIMD the extra spills cost more.)
 template <int KLO, int KHI = 5>
-BNS_FDI_ATTR void fdi_bundle(double y, double* F) {
+BNS_HD void fdi_bundle(double y, double* F) {
   double x = -fabs(y);
   bool ebr = x < -2.;
   bool rbr = !ebr && (x <= 0.);  // NaN arguments give 0, as the reference

@@ -2,6 +2,7 @@
 #include "gray_tables.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <numeric>
 #include <vector>
@@ -91,7 +92,11 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
   // weight folds w_i w_j, the Jacobian x_i or 1/x_i^3
   // (integration.hpp:730-756), nu^2 nubar^2 / T^4, and the factor
   // (1/2) t_neps^2 / t_pair^2 = 8 that puts both 2D sums on one scale.
+  // The half-cells are stored in ascending order of w = min(w, w') so that
+  // the 64 lanes of a round hold similar energies and take the same branches
+  // (Fermi-integral rational vs exponential form, NES/NPS Taylor forms).
   const double tn = 4.0 * c::PairScale;
+  std::vector<std::array<double, NP_REC>> rec(size_t(2) * n * n);
   int h = 0;
   for (int i = 0; i < n; ++i) {
     for (int j = 0; j < n; ++j) {
@@ -107,7 +112,7 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
           wp = 0.5 * tn * (1.0 + xj) / xi;
           jac = 1.0 / (xi * xi * xi);
         }
-        double* r = block + L.np + NP_REC * h;
+        double* r = rec[h].data();
         r[NP_W] = w;
         r[NP_WP] = wp;
         r[NP_AD] = std::fabs(w - wp);
@@ -123,6 +128,12 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
       }
     }
   }
+  std::stable_sort(rec.begin(), rec.end(),
+                   [](const std::array<double, NP_REC>& a, const std::array<double, NP_REC>& b) {
+                     return a[NP_W] < b[NP_W];
+                   });
+  for (size_t k = 0; k < rec.size(); ++k)
+    std::copy(rec[k].begin(), rec[k].end(), block + L.np + NP_REC * k);
   return L;
 }
 

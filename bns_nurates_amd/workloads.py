@@ -188,6 +188,21 @@ def make_points(npts: int, seed: int = 0, rho_range=(1e6, 1e15), T_range=(0.5, 5
     return pts
 
 
+def make_tail_points(npts: int, seed: int = 0) -> np.ndarray:
+    """Cold, degenerate points of the same distributions (T < 1 MeV and
+    eta_e = mu_e / T > 50): the tail where reduced precision is most exposed.
+    Draws T in [0.5, 1] MeV and rho in [1e12, 1e15] g/cm^3 and keeps the
+    first npts points with eta_e > 50."""
+    out, got, k = [], 0, 0
+    while got < npts:
+        p = make_points(4 * npts, seed=seed + 7919 * k, rho_range=(1e12, 1e15), T_range=(0.5, 1.0))
+        keep = p[:, p[P_MUE] / p[P_T] > 50.0]
+        out.append(keep)
+        got += keep.shape[1]
+        k += 1
+    return np.ascontiguousarray(np.concatenate(out, axis=1)[:, :npts])
+
+
 # The named workloads (sizes, quadrature order, reactions, bremsstrahlung)
 WORKLOADS = {
     "pts4096_n10_brt06": dict(npts=4096, nquad=10, brem=BREM_BRT06, seed=0),
