@@ -77,6 +77,123 @@ def mixed_error_stats(got, ref, mag):
             "fp64_determined_frac": det_frac}
 
 
+# request mix of the serving workload: meshblock sizes and weights, share of
+# single-point calls, quadrature orders, precisions
+SERVE_SIZES = [8 ** 3, 16 ** 3, 32 ** 3, 64 ** 3]
+SERVE_WEIGHTS = [0.4, 0.3, 0.2, 0.1]
+SERVE_SINGLE = 0.05
+SERVE_ORDERS = [8, 10, 16]
+
+
+def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
+    """Poisson arrivals of opacity requests at several rates (fractions of
+    the closed-loop capacity) through the continuous-batching server; p50/p99
+    latency and served points/s per rate."""
+    import threading
+    from collections import deque
+    from bns_nurates_amd import api
+    from bns_nurates_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    # request templates: own inputs (slices of one seeded pool), pinned
+    sizes = [1 if rng.random() < SERVE_SINGLE else int(rng.choice(SERVE_SIZES, p=SERVE_WEIGHTS))
+             for _ in range(templates)]
+    pool = W.make_points(max(sizes) * 2 + sum(sizes) // 4, seed=seed)
+    tpl = []
+    for i, n in enumerate(sizes):
+        a = api.PinnedBuffer((21, n))
+        o = api.PinnedBuffer((20, n))
+        off = int(rng.integers(0, pool.shape[1] - n + 1))
+        a.array[...] = pool[:, off:off + n]
+        tpl.append(dict(n=n, pts=a, out=o, nquad=int(rng.choice(SERVE_ORDERS)),
+                        precision=int(rng.integers(0, 2))))
+    prm = W.params_default_all(W.BREM_BRT06)
+    srv = api.Server(0, hbm_budget_bytes=4 << 30, streams=4, max_batch_points=1 << 20)
+
+    def submit(t):
+        return srv.submit(t["pts"].array, t["out"].array, t["nquad"], W.FLAGS_ALL, prm,
+                          precision=t["precision"])
+
+    # warm-up (table builds, first launches) and closed-loop capacity
+    for t in tpl:
+        srv.wait(submit(t))
+    t0 = time.perf_counter()
+    for _ in range(2):
+        ids = [submit(t) for t in tpl]
+        for i in ids:
+            srv.wait(i)
+    cap_pts = 2 * sum(sizes) / (time.perf_counter() - t0)
+    mean_pts = sum(sizes) / len(sizes)
+    cap_rps = cap_pts / mean_pts
+    log(f"serving: closed-loop capacity {cap_pts:.3g} points/s = {cap_rps:.0f} requests/s")
+    rows = []
+    for frac in fractions:
+        rate = frac * cap_rps
+        free = deque(range(len(tpl)))
+        inflight = deque()
+        lock = threading.Condition()
+        lat, served = [], [0]
+        stop = [False]
+        b0 = srv.stats()["batches"]
+
+        def waiter():
+            while True:
+                with lock:
+                    while not inflight and not stop[0]:
+                        lock.wait()
+                    if not inflight and stop[0]:
+                        return
+                    rid, k = inflight.popleft()
+                lat.append(srv.wait(rid))
+                with lock:
+                    served[0] += tpl[k]["n"]
+                    free.append(k)
+                    lock.notify_all()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_start = time.perf_counter()
+        t_next = t_start
+        n_sub = 0
+        while t_next - t_start < seconds:
+            now = time.perf_counter()
+            if now < t_next:
+                time.sleep(min(t_next - now, 1e-3))
+                continue
+            with lock:
+                while not free:
+                    lock.wait()
+                k = free.popleft()
+                inflight.append((submit(tpl[k]), k))
+                lock.notify_all()
+            n_sub += 1
+            t_next += rng.exponential(1.0 / rate)
+        with lock:
+            stop[0] = True
+            lock.notify_all()
+        th.join()
+        wall = time.perf_counter() - t_start
+        lat_ms = np.array(lat) * 1e3
+        nb = srv.stats()["batches"] - b0
+        rows.append({"arrival_rate_rps": round(rate, 1), "requests": n_sub,
+                     "p50_ms": float(f"{np.percentile(lat_ms, 50):.3g}"),
+                     "p99_ms": float(f"{np.percentile(lat_ms, 99):.3g}"),
+                     "points_per_s": float(f"{served[0] / wall:.4g}"),
+                     "requests_per_batch": float(f"{n_sub / max(nb, 1):.3g}")})
+        log(f"serving at {rate:.0f} req/s: {rows[-1]}")
+    st = srv.stats()
+    srv.close()
+    for t in tpl:
+        t["pts"].close()
+        t["out"].close()
+    return {"workload": "Poisson arrivals; meshblocks 8^3/16^3/32^3/64^3 (0.4/0.3/0.2/0.1) "
+                        "+ 5% single points; n in {8,10,16}; FP64 or mixed per request; "
+                        "host inputs in pinned memory (PCIe copies included in latency)",
+            "capacity_points_per_s": float(f"{cap_pts:.4g}"), "sweep": rows,
+            "server": {"streams": 4, "hbm_budget_bytes": st["arena_bytes"],
+                       "max_inflight_bytes": st["max_inflight_bytes"],
+                       "max_concurrent_batches": st["max_concurrent_batches"]}}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -114,6 +231,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-hr98", action="store_true", help="skip the HR98 passes")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision passes")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -247,6 +365,8 @@ def main():
             "speedup_vs_fp64": res[brem]["kernel_ms"] / h["kernel_ms"] if brem in res else None,
             "fp32_parts": "distribution values (exp/log), HR98 channel functions",
             "rel_err_vs_fp64_oracle": acc}
+    if not args.no_serving:
+        line["serving"] = serving_sweep()
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(pts, args.nquad, W.params_default_all(W.BREM_BRT06))

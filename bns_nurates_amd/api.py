@@ -117,6 +117,20 @@ def as_params(params) -> OpacityParams:
     return OpacityParams(*a)
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget_bytes", C.c_size_t), ("streams", C.c_int),
+                ("max_batch_points", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "requests_submitted", "requests_completed", "points_completed", "batches",
+        "max_inflight_bytes", "max_concurrent_batches", "arena_bytes")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -146,6 +160,16 @@ def load_library():
         "bnsamd_distr_from_m1": ([vp, i64, i64, vp, vp], C.c_int),
         "bnsamd_equilibrium": ([vp, i64, i64, vp, vp], C.c_int),
         "bnsamd_fdi": ([vp, C.c_int, i64, vp, vp], C.c_int),
+        "bnsamd_server_create": ([C.POINTER(_ServerConfig), C.POINTER(vp)], C.c_int),
+        "bnsamd_server_destroy": ([vp], None),
+        "bnsamd_server_last_error": ([vp], C.c_char_p),
+        "bnsamd_server_submit": ([vp, i64, vp, C.c_int, C.POINTER(_Flags), C.POINTER(_Params),
+                                  C.c_int, C.c_int, vp, C.POINTER(i64)], C.c_int),
+        "bnsamd_server_poll": ([vp, i64, C.POINTER(C.c_int)], C.c_int),
+        "bnsamd_server_wait": ([vp, i64, C.POINTER(C.c_double)], C.c_int),
+        "bnsamd_server_get_stats": ([vp, C.POINTER(ServerStats)], C.c_int),
+        "bnsamd_host_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
+        "bnsamd_host_free": ([vp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -317,3 +341,102 @@ class Context:
         self._check(self.lib.bnsamd_fdi(self.h, int(k), y.numel(), y.data_ptr(), out.data_ptr()),
                     "bnsamd_fdi")
         return out
+
+
+class PinnedBuffer:
+    """Page-locked host memory (bnsamd_host_alloc) viewed as a float64 numpy
+    array of the given shape; freed by close() or when collected."""
+
+    def __init__(self, shape):
+        self.lib = load_library()
+        n = int(np.prod(shape))
+        p = C.c_void_p()
+        rc = self.lib.bnsamd_host_alloc(C.c_size_t(max(n, 1) * 8), C.byref(p))
+        if rc:
+            raise BnsAmdError(f"bnsamd_host_alloc({n * 8} bytes): {_STATUS.get(rc, rc)}")
+        self.ptr = p
+        self.array = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(max(n, 1),))
+        self.array = self.array[:n].reshape(shape)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            self.lib.bnsamd_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request server (bnsamd_server_*): submit host SoA
+    point blocks [21, npts] from any thread, wait for their [20, npts]
+    outputs by id."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, streams: int = 4,
+                 max_batch_points: int = 1 << 20):
+        self.lib = load_library()
+        cfg = _ServerConfig(int(device), int(hbm_budget_bytes), int(streams), int(max_batch_points))
+        h = C.c_void_p()
+        rc = self.lib.bnsamd_server_create(C.byref(cfg), C.byref(h))
+        if rc:
+            raise BnsAmdError(f"bnsamd_server_create: {_STATUS.get(rc, rc)}")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bnsamd_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, pts: np.ndarray, out: np.ndarray, nquad: int, flags=None, params=None,
+               stim_abs: int = 1, precision: int = FP64) -> int:
+        """Queue a request; pts [21, npts] and out [20, npts] C-contiguous
+        float64 (pinned buffers let the copies overlap).  Both must stay
+        alive until wait() returns for the id."""
+        if pts.dtype != np.float64 or not pts.flags.c_contiguous or pts.shape[0] != 21:
+            raise BnsAmdError("submit: pts must be a C-contiguous float64 [21, npts] array")
+        npts = pts.shape[1]
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.shape != (20, npts):
+            raise BnsAmdError("submit: out must be a C-contiguous float64 [20, npts] array")
+        f, p = as_flags(flags).c(), as_params(params).c()
+        rid = C.c_int64()
+        rc = self.lib.bnsamd_server_submit(self.h, npts, pts.ctypes.data, int(nquad), C.byref(f),
+                                           C.byref(p), int(stim_abs), int(precision),
+                                           out.ctypes.data, C.byref(rid))
+        if rc:
+            msg = self.lib.bnsamd_server_last_error(self.h).decode(errors="replace")
+            raise BnsAmdError(f"bnsamd_server_submit: {_STATUS.get(rc, rc)}: {msg}")
+        self._keep[rid.value] = (pts, out)
+        return rid.value
+
+    def poll(self, rid: int) -> bool:
+        d = C.c_int()
+        rc = self.lib.bnsamd_server_poll(self.h, int(rid), C.byref(d))
+        if rc:
+            raise BnsAmdError(f"bnsamd_server_poll: {_STATUS.get(rc, rc)}")
+        return bool(d.value)
+
+    def wait(self, rid: int) -> float:
+        """Blocks until the request is done; returns its latency in seconds."""
+        lat = C.c_double()
+        rc = self.lib.bnsamd_server_wait(self.h, int(rid), C.byref(lat))
+        self._keep.pop(int(rid), None)
+        if rc:
+            raise BnsAmdError(f"bnsamd_server_wait({rid}): {_STATUS.get(rc, rc)}")
+        return lat.value * 1e-6
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self.lib.bnsamd_server_get_stats(self.h, C.byref(st))
+        return st.as_dict()

@@ -76,11 +76,6 @@ int download(bnsamd_context* c, double* h, const double* d, size_t n, const char
   return e == hipSuccess ? BNSAMD_OK : hip_fail(c, e, who);
 }
 
-double gamma_stirling_36() {
-  // GammaStirling (functions.hpp:4460-4470) at c_f + 3 = 3.6
-  const double x = 0.6 + 3.0;
-  return std::sqrt(6.283185307179586 / x) * std::pow(x / 2.718281828459045235360287471352, x);
-}
 
 // ---- small elementwise kernels ------------------------------------------------
 template <int K>
@@ -213,7 +208,7 @@ int run_gray(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pts,
                               params->brem_implementation, params->use_NN_medium_corr,
                               params->neglect_blocking};
   a.stim = stim_abs;
-  a.gamma36 = gamma_stirling_36();
+  a.gamma36 = bns::gamma_stirling_36();
   hipError_t e;
   if (precision == BNSAMD_FP64) {
     e = bns::launch_gray_m1_f64(a, c->stream);
@@ -401,7 +396,7 @@ int bnsamd_distr_from_m1(bnsamd_context* c, int64_t npts, int64_t ld, const doub
   (void)hipSetDevice(c->device);
   unsigned blocks = unsigned((npts + 255) / 256);
   hipLaunchKernelGGL(k_distr_from_m1, dim3(blocks), dim3(256), 0, c->stream, npts, ld, d_pts,
-                     d_distr, gamma_stirling_36());
+                     d_distr, bns::gamma_stirling_36());
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? BNSAMD_OK : hip_fail(c, e, "k_distr_from_m1");
 }
@@ -456,7 +451,7 @@ static int distr_host(bnsamd_context* c, int op, int64_t npts, double* h_pts, do
   unsigned blocks = unsigned((npts + 255) / 256);
   if (op == 0)
     hipLaunchKernelGGL(k_distr_from_m1, dim3(blocks), dim3(256), 0, c->stream, npts, npts, dp,
-                       dd, gamma_stirling_36());
+                       dd, bns::gamma_stirling_36());
   else if (op == 1)
     hipLaunchKernelGGL(k_equilibrium, dim3(blocks), dim3(256), 0, c->stream, npts, npts, dp, dd);
   else

@@ -137,4 +137,9 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
   return L;
 }
 
+double gamma_stirling_36() {
+  const double x = 0.6 + 3.0;
+  return std::sqrt(6.283185307179586 / x) * std::pow(x / 2.718281828459045235360287471352, x);
+}
+
 }  // namespace bns

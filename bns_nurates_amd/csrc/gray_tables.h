@@ -71,6 +71,10 @@ TableLayout gray_table_layout(int n1, int n2);
 TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2,
                               const double* x2, const double* w2, double* block);
 
+// GammaStirling (functions.hpp:4460-4470) at c_f + 3 = 3.6, the Gamma
+// factor of the optically thin distribution normalisation
+double gamma_stirling_36();
+
 }  // namespace bns
 
 #endif  // BNS_NURATES_AMD_GRAY_TABLES_H_

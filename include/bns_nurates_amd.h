@@ -154,6 +154,56 @@ int bnsamd_m1_densities_eq_host(bnsamd_context* ctx, int64_t npts, const double*
  * functions.hpp:1551-2837), elementwise on device arrays. */
 int bnsamd_fdi(bnsamd_context* ctx, int k, int64_t n, const double* d_y, double* d_out);
 
+/* ---- serving --------------------------------------------------------------
+ * A request server for many concurrent callers (meshblocks of uneven sizes):
+ * requests are queued, admitted against a fixed device-memory budget,
+ * batched with other pending requests of the same quadrature order,
+ * precision, flags and parameters, and run on several streams so that their
+ * work overlaps.  A request's results do not depend on its batch.  No
+ * counterpart in the reference, whose callers invoke ComputeM1Opacities per
+ * point inside their own loops (tests/tests.hpp:272-274,756); each request is
+ * the batched equivalent of such a loop, with the distribution reconstructed
+ * from (n, J, chi) as CalculateDistrParamsFromM1 does
+ * (distribution.hpp:323-333) and the same quadrature for quad_1d and quad_2d,
+ * as all of the reference's callers pass. */
+typedef struct bnsamd_server bnsamd_server;
+
+typedef struct bnsamd_server_config {
+  int device;               /* HIP device */
+  size_t hbm_budget_bytes;  /* device arena for in-flight batches: 328 B per point */
+  int streams;              /* batches in flight at once, 1..16 */
+  int64_t max_batch_points; /* points per launch (a larger single request runs alone) */
+} bnsamd_server_config;
+
+typedef struct bnsamd_server_stats {
+  int64_t requests_submitted, requests_completed, points_completed, batches;
+  int64_t max_inflight_bytes, max_concurrent_batches, arena_bytes;
+} bnsamd_server_stats;
+
+int bnsamd_server_create(const bnsamd_server_config* cfg, bnsamd_server** out);
+/* Finishes every submitted request, then releases the server.  No call may
+ * be waiting on the server when it is destroyed. */
+void bnsamd_server_destroy(bnsamd_server* s);
+const char* bnsamd_server_last_error(const bnsamd_server* s);
+/* Queues npts points (host SoA block h_pts[21][npts]); the outputs
+ * h_out[20][npts] are written when the request completes.  Both buffers must
+ * stay valid until bnsamd_server_wait returns for this request; pinned
+ * buffers (bnsamd_host_alloc) let the copies run asynchronously.  Rejects a
+ * request larger than the server's HBM budget. */
+int bnsamd_server_submit(bnsamd_server* s, int64_t npts, const double* h_pts, int nquad,
+                         const bnsamd_flags* flags, const bnsamd_params* params, int stim_abs,
+                         int precision, double* h_out, int64_t* request_id);
+int bnsamd_server_poll(bnsamd_server* s, int64_t request_id, int* done);
+/* Blocks until the request has completed; returns its status and, if
+ * latency_us is not NULL, the time from submission to completion.  The id is
+ * released afterwards. */
+int bnsamd_server_wait(bnsamd_server* s, int64_t request_id, double* latency_us);
+int bnsamd_server_get_stats(bnsamd_server* s, bnsamd_server_stats* out);
+
+/* Pinned (page-locked) host memory for request buffers. */
+int bnsamd_host_alloc(size_t bytes, void** ptr);
+void bnsamd_host_free(void* ptr);
+
 #ifdef __cplusplus
 }
 #endif
