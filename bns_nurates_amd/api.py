@@ -170,6 +170,17 @@ def load_library():
         "bnsamd_server_get_stats": ([vp, C.POINTER(ServerStats)], C.c_int),
         "bnsamd_host_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
         "bnsamd_host_free": ([vp], None),
+        "bnsamd_mlp_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
+        "bnsamd_mlp_destroy": ([vp], None),
+        "bnsamd_mlp_last_error": ([vp], C.c_char_p),
+        "bnsamd_mlp_param_count": ([], C.c_int),
+        "bnsamd_mlp_set_params": ([vp, C.c_int, vp], C.c_int),
+        "bnsamd_mlp_get_params": ([vp, C.c_int, vp], C.c_int),
+        "bnsamd_mlp_set_norm": ([vp, vp], C.c_int),
+        "bnsamd_mlp_gradients": ([vp, C.c_int, C.c_int, vp, vp, vp, vp], C.c_int),
+        "bnsamd_mlp_train": ([vp, C.c_int, i64, vp, vp, vp, C.c_int, C.c_int, C.c_float,
+                              C.c_float, vp], C.c_int),
+        "bnsamd_mlp_predict": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

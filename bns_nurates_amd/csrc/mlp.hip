@@ -1,0 +1,517 @@
+// mlp.hip -- BF16-MFMA multilayer perceptron (ensemble) for the learned form
+// of the gray M1 opacities: training kernels (GEMM, bias/SiLU forward and
+// backward, MSE gradient, Adam), the native training loop, and the fused
+// inference kernel that evaluates every ensemble member with activations in
+// LDS.  No counterpart in the reference.
+//
+// Network of one member: 12 input features (padded to K = 32) -> 256 -> 256
+// -> 256 -> 256 -> 15 outputs (padded to N = 16), SiLU on the hidden layers.
+// Weights W_l [K_l][N_l] and biases b_l [N_l] are FP32 master copies; the
+// matrix products take BF16 weights and activations and accumulate in FP32
+// on the matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// MFMA 16x16x32 operand layout used here (64 lanes): A fragment of a 16 x 32
+// tile, lane l holds row (l % 16), eight consecutive k starting at
+// 8 (l / 16); B fragment of a 32 x 16 tile, lane l holds column (l % 16) at
+// the same eight k; the 16 x 16 result, lane l holds rows 4 (l / 16) + v,
+// v = 0..3, of column (l % 16).  (Any k order shared by A and B gives the
+// same product.)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bns_nurates_amd.h"
+#include "mlp.h"
+
+namespace bns {
+namespace mlp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+  uint32_t u = uint32_t(h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+__device__ inline bf16x8 ld8(const uint16_t* p) {  // 16-byte aligned LDS or global read
+  uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ inline float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ inline float silu_grad(float z) {
+  float s = 1.f / (1.f + __expf(-z));
+  return s * (1.f + z * (1.f - s));
+}
+
+// ---- training kernels ---------------------------------------------------------
+// C[m][n] (+)= sum_k A(m,k) B(k,n), A(m,k) = A[m sam + k sak],
+// B(k,n) = B[k sbk + n sbn] (BF16), C FP32 with leading dimension ldc.
+// 64 x 64 tile per workgroup, 4 waves of 32 x 32, K in steps of 32; gridDim.z
+// splits K into chunks of kchunk (ATOMIC: the chunks add into C).
+template <bool ATOMIC>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t* __restrict__ A,
+                                              int64_t sam, int64_t sak,
+                                              const uint16_t* __restrict__ B, int64_t sbk,
+                                              int64_t sbn, float* __restrict__ C, int64_t ldc,
+                                              int kchunk) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];  // [m][k]
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];  // [n][k]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      int e = t + i * 256;
+      int mm = sak == 1 ? e >> 5 : e & 63, kk = sak == 1 ? e & 31 : e >> 6;
+      int gm = m0 + mm, gk = k0 + kk;
+      As[mm][kk] = (gm < M && gk < kend) ? A[gm * sam + gk * sak] : uint16_t(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      int e = t + i * 256;
+      int nn = sbn == 1 ? e & 63 : e >> 5, kk = sbn == 1 ? e >> 6 : e & 31;
+      int gn = n0 + nn, gk = k0 + kk;
+      Bs[nn][kk] = (gn < N && gk < kend) ? B[gk * sbk + gn * sbn] : uint16_t(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      bf16x8 a = ld8(&As[wm + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        bf16x8 b = ld8(&Bs[wn + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v, col = n0 + wn + j * 16 + (lane & 15);
+        if (row < M && col < N) {
+          if (ATOMIC) atomicAdd(&C[row * ldc + col], acc[i][j][v]);
+          else C[row * ldc + col] = acc[i][j][v];
+        }
+      }
+}
+
+// Z += b (kept as the pre-activation), H = BF16(act(Z)); one thread per
+// column, 64 rows per workgroup
+__global__ void k_bias_act(int rows, int cols, float* __restrict__ Z, const float* __restrict__ b,
+                           int act, uint16_t* __restrict__ H) {
+  int c = threadIdx.x, r0 = blockIdx.x * 64;
+  if (c >= cols) return;
+  float bc = b[c];
+  for (int r = r0; r < min(rows, r0 + 64); ++r) {
+    float z = Z[int64_t(r) * cols + c] + bc;
+    Z[int64_t(r) * cols + c] = z;
+    if (H) H[int64_t(r) * cols + c] = f2bf(act ? silu(z) : z);
+  }
+}
+
+// dZ = dH act'(Z) as BF16, db += column sums of dZ
+__global__ void k_act_bwd(int rows, int cols, const float* __restrict__ dH,
+                          const float* __restrict__ Z, uint16_t* __restrict__ dZ,
+                          float* __restrict__ db) {
+  int c = threadIdx.x, r0 = blockIdx.x * 64;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = r0; r < min(rows, r0 + 64); ++r) {
+    int64_t i = int64_t(r) * cols + c;
+    float d = dH[i] * silu_grad(Z[i]);
+    dZ[i] = f2bf(d);
+    s += d;
+  }
+  atomicAdd(&db[c], s);
+}
+
+// last layer: y = Z (bias added), e = y - t on the first nout columns;
+// loss += sum e^2, dZ = 2 e / (rows nout) as BF16, db += column sums
+__global__ void k_mse_grad(int rows, int cols, int nout, const float* __restrict__ Z,
+                           const float* __restrict__ T, uint16_t* __restrict__ dZ,
+                           float* __restrict__ db, float* __restrict__ loss) {
+  __shared__ float red[256];
+  int c = threadIdx.x % cols, rl = threadIdx.x / cols, rstep = blockDim.x / cols;
+  int r0 = blockIdx.x * 256;
+  float s = 0.f, l2 = 0.f;
+  const float scale = 2.f / float(rows * nout);
+  if (rl < rstep) {
+    for (int r = r0 + rl; r < min(rows, r0 + 256); r += rstep) {
+      int64_t i = int64_t(r) * cols + c;
+      float e = c < nout ? Z[i] - T[i] : 0.f;
+      float d = e * scale;
+      dZ[i] = f2bf(d);
+      s += d;
+      l2 += e * e;
+    }
+  }
+  red[threadIdx.x] = l2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+  if (rl < rstep) atomicAdd(&db[c], s);
+}
+
+// Adam step over the whole parameter blob; refreshes the BF16 copy
+__global__ void k_adam(int64_t n, float* __restrict__ w, const float* __restrict__ g,
+                       float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ wb,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  w[i] = wi;
+  wb[i] = f2bf(wi);
+}
+
+// batch assembly: rows perm[start + r] of the normalised features (FP32,
+// [n][12]) and targets ([n][15]) into the padded BF16 input [B][32] and FP32
+// targets [B][16]
+__global__ void k_gather_batch(int B, const int32_t* __restrict__ perm, int64_t start,
+                               const float* __restrict__ X, const float* __restrict__ Y,
+                               uint16_t* __restrict__ Xb, float* __restrict__ Tb) {
+  int r = blockIdx.x * blockDim.y + threadIdx.y, c = threadIdx.x;  // 32 x 8
+  if (r >= B) return;
+  int64_t src = perm[start + r];
+  Xb[int64_t(r) * kK0 + c] = c < kFeat ? f2bf(X[src * kFeat + c]) : uint16_t(0);
+  if (Y && c < kNL) Tb[int64_t(r) * kNL + c] = c < kOut ? Y[src * kOut + c] : 0.f;
+}
+
+__global__ void k_fill(int64_t n, float* __restrict__ p, float v) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace mlp
+}  // namespace bns
+
+using namespace bns::mlp;
+
+// ---- the MLP object -------------------------------------------------------------
+struct bnsamd_mlp {
+  int device = 0, members = 0, maxb = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // per member: FP32 parameters, gradients, Adam moments, BF16 copy, packed
+  // BF16 weights for the inference kernel
+  std::vector<float*> w, g, m, v;
+  std::vector<uint16_t*> wb, wp;
+  std::vector<int64_t> steps;
+  float* norm = nullptr;  // device copy of the normalisation block
+  bnsamd_mlp_norm hnorm{};
+  // training workspace (maxb rows)
+  uint16_t* Xb = nullptr;
+  float* Tb = nullptr;
+  float* Z[kLayers] = {};
+  uint16_t* H[kLayers] = {};
+  uint16_t* dZ[kLayers] = {};
+  float* dH = nullptr;
+  float* loss = nullptr;
+};
+
+namespace {
+
+int mlp_fail(bnsamd_mlp* m, int code, const std::string& msg) {
+  if (m) m->err = msg;
+  return code;
+}
+
+void gemm(hipStream_t s, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak,
+          const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, int splitk) {
+  int kchunk = (K + splitk - 1) / splitk;
+  kchunk = (kchunk + 31) / 32 * 32;
+  int z = (K + kchunk - 1) / kchunk;
+  dim3 grid((N + 63) / 64, (M + 63) / 64, z), block(256);
+  if (z > 1)
+    hipLaunchKernelGGL(k_gemm<true>, grid, block, 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
+                       kchunk);
+  else
+    hipLaunchKernelGGL(k_gemm<false>, grid, block, 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C,
+                       ldc, kchunk);
+}
+
+void fill(hipStream_t s, int64_t n, float* p, float v) {
+  hipLaunchKernelGGL(k_fill, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, p, v);
+}
+
+// forward pass of one member on B rows of Xb (BF16 [B][32]); Z[l] hold the
+// pre-activations, H[l] the BF16 activations
+void forward(bnsamd_mlp* m, int e, int B) {
+  hipStream_t s = m->stream;
+  const uint16_t* in = m->Xb;
+  for (int l = 0; l < kLayers; ++l) {
+    const int K = kLayerK[l], N = kLayerN[l];
+    gemm(s, B, N, K, in, K, 1, m->wb[e] + kWOff[l], N, 1, m->Z[l], N, 1);
+    hipLaunchKernelGGL(k_bias_act, dim3((B + 63) / 64), dim3(256), 0, s, B, N, m->Z[l],
+                       m->w[e] + kBOff[l], l + 1 < kLayers ? 1 : 0,
+                       l + 1 < kLayers ? m->H[l] : static_cast<uint16_t*>(nullptr));
+    in = m->H[l];
+  }
+}
+
+// loss, gradients of one member on B rows (Xb, Tb) into g[e]
+void backward(bnsamd_mlp* m, int e, int B) {
+  hipStream_t s = m->stream;
+  fill(s, kParams, m->g[e], 0.f);
+  const int L = kLayers - 1;
+  hipLaunchKernelGGL(k_mse_grad, dim3((B + 255) / 256), dim3(256), 0, s, B, kNL, kOut, m->Z[L],
+                     m->Tb, m->dZ[L], m->g[e] + kBOff[L], m->loss);
+  for (int l = L; l >= 0; --l) {
+    const int K = kLayerK[l], N = kLayerN[l];
+    const uint16_t* Hin = l ? m->H[l - 1] : m->Xb;
+    // dW = H_in^T dZ (reduction over the batch, split into 512-row chunks)
+    gemm(s, K, N, B, Hin, 1, K, m->dZ[l], N, 1, m->g[e] + kWOff[l], N, (B + 511) / 512);
+    if (l) {
+      // dH_in = dZ W^T, then through the SiLU of layer l - 1
+      gemm(s, B, K, N, m->dZ[l], N, 1, m->wb[e] + kWOff[l], 1, N, m->dH, K, 1);
+      hipLaunchKernelGGL(k_act_bwd, dim3((B + 63) / 64), dim3(256), 0, s, B, K, m->dH,
+                         m->Z[l - 1], m->dZ[l - 1], m->g[e] + kBOff[l - 1]);
+    }
+  }
+}
+
+void adam(bnsamd_mlp* m, int e, float lr) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int64_t t = ++m->steps[e];
+  float c1 = 1.f / (1.f - std::pow(b1, float(t))), c2 = 1.f / (1.f - std::pow(b2, float(t)));
+  hipLaunchKernelGGL(k_adam, dim3(unsigned((kParams + 255) / 256)), dim3(256), 0, m->stream,
+                     int64_t(kParams), m->w[e], m->g[e], m->m[e], m->v[e], m->wb[e], lr, b1, b2,
+                     eps, c1, c2);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bnsamd_mlp_create(int device, int members, int max_batch, bnsamd_mlp** out) {
+  if (!out || members < 1 || members > kMaxMembers || max_batch < 64 || max_batch > (1 << 16))
+    return BNSAMD_ERR_INVALID;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return BNSAMD_ERR_NO_DEVICE;
+  auto* m = new bnsamd_mlp();
+  m->device = device;
+  m->members = members;
+  m->maxb = max_batch;
+  bool ok = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
+  auto alloc = [&](void** p, size_t bytes) {
+    ok = ok && hipMalloc(p, bytes) == hipSuccess && hipMemset(*p, 0, bytes) == hipSuccess;
+  };
+  for (int e = 0; e < members && ok; ++e) {
+    float *w, *g, *mm, *v;
+    uint16_t *wb, *wp;
+    alloc(reinterpret_cast<void**>(&w), sizeof(float) * kParams);
+    alloc(reinterpret_cast<void**>(&g), sizeof(float) * kParams);
+    alloc(reinterpret_cast<void**>(&mm), sizeof(float) * kParams);
+    alloc(reinterpret_cast<void**>(&v), sizeof(float) * kParams);
+    alloc(reinterpret_cast<void**>(&wb), sizeof(uint16_t) * kParams);
+    alloc(reinterpret_cast<void**>(&wp), sizeof(uint16_t) * kPacked);
+    m->w.push_back(w);
+    m->g.push_back(g);
+    m->m.push_back(mm);
+    m->v.push_back(v);
+    m->wb.push_back(wb);
+    m->wp.push_back(wp);
+    m->steps.push_back(0);
+  }
+  const size_t B = size_t(max_batch);
+  alloc(reinterpret_cast<void**>(&m->norm), sizeof(bnsamd_mlp_norm));
+  alloc(reinterpret_cast<void**>(&m->Xb), sizeof(uint16_t) * B * kK0);
+  alloc(reinterpret_cast<void**>(&m->Tb), sizeof(float) * B * kNL);
+  for (int l = 0; l < kLayers; ++l) {
+    alloc(reinterpret_cast<void**>(&m->Z[l]), sizeof(float) * B * kLayerN[l]);
+    alloc(reinterpret_cast<void**>(&m->H[l]), sizeof(uint16_t) * B * kLayerN[l]);
+    alloc(reinterpret_cast<void**>(&m->dZ[l]), sizeof(uint16_t) * B * kLayerN[l]);
+  }
+  alloc(reinterpret_cast<void**>(&m->dH), sizeof(float) * B * kHidden);
+  alloc(reinterpret_cast<void**>(&m->loss), sizeof(float));
+  if (!ok) {
+    bnsamd_mlp_destroy(m);
+    return BNSAMD_ERR_NOMEM;
+  }
+  *out = m;
+  return BNSAMD_OK;
+}
+
+void bnsamd_mlp_destroy(bnsamd_mlp* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  auto fr = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  for (auto* p : m->w) fr(p);
+  for (auto* p : m->g) fr(p);
+  for (auto* p : m->m) fr(p);
+  for (auto* p : m->v) fr(p);
+  for (auto* p : m->wb) fr(p);
+  for (auto* p : m->wp) fr(p);
+  fr(m->norm);
+  fr(m->Xb);
+  fr(m->Tb);
+  for (int l = 0; l < kLayers; ++l) {
+    fr(m->Z[l]);
+    fr(m->H[l]);
+    fr(m->dZ[l]);
+  }
+  fr(m->dH);
+  fr(m->loss);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+const char* bnsamd_mlp_last_error(const bnsamd_mlp* m) { return m ? m->err.c_str() : ""; }
+
+int bnsamd_mlp_param_count(void) { return kParams; }
+
+int bnsamd_mlp_set_params(bnsamd_mlp* m, int member, const float* h_params) {
+  if (!m || member < 0 || member >= m->members || !h_params) return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  std::vector<uint16_t> hb(kParams);
+  for (int i = 0; i < kParams; ++i) hb[i] = f2bf(h_params[i]);
+  if (hipMemcpy(m->w[member], h_params, sizeof(float) * kParams, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(m->wb[member], hb.data(), sizeof(uint16_t) * kParams, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemset(m->m[member], 0, sizeof(float) * kParams) != hipSuccess ||
+      hipMemset(m->v[member], 0, sizeof(float) * kParams) != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_set_params: copy");
+  m->steps[member] = 0;
+  return BNSAMD_OK;
+}
+
+int bnsamd_mlp_get_params(bnsamd_mlp* m, int member, float* h_params) {
+  if (!m || member < 0 || member >= m->members || !h_params) return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  if (hipStreamSynchronize(m->stream) != hipSuccess ||
+      hipMemcpy(h_params, m->w[member], sizeof(float) * kParams, hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_get_params: copy");
+  return BNSAMD_OK;
+}
+
+int bnsamd_mlp_set_norm(bnsamd_mlp* m, const bnsamd_mlp_norm* norm) {
+  if (!m || !norm) return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  m->hnorm = *norm;
+  if (hipMemcpy(m->norm, norm, sizeof(*norm), hipMemcpyHostToDevice) != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_set_norm: copy");
+  return BNSAMD_OK;
+}
+
+int bnsamd_mlp_gradients(bnsamd_mlp* m, int member, int rows, const float* d_x, const float* d_y,
+                         float* h_grad, float* h_loss) {
+  if (!m || member < 0 || member >= m->members || rows < 1 || rows > m->maxb || !d_x || !d_y ||
+      !h_grad)
+    return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  std::vector<int32_t> ident(rows);
+  for (int i = 0; i < rows; ++i) ident[i] = i;
+  int32_t* perm = nullptr;
+  if (hipMalloc(&perm, sizeof(int32_t) * rows) != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_gradients: hipMalloc");
+  hipError_t e = hipMemcpy(perm, ident.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice);
+  fill(m->stream, 1, m->loss, 0.f);
+  hipLaunchKernelGGL(k_gather_batch, dim3((rows + 7) / 8), dim3(32, 8), 0, m->stream, rows, perm,
+                     int64_t(0), d_x, d_y, m->Xb, m->Tb);
+  forward(m, member, rows);
+  backward(m, member, rows);
+  float loss = 0.f;
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(h_grad, m->g[member], sizeof(float) * kParams, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&loss, m->loss, sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(perm);
+  if (e != hipSuccess) return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_gradients");
+  if (h_loss) *h_loss = loss / float(rows * kOut);
+  return BNSAMD_OK;
+}
+
+int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, const float* d_y,
+                     const int32_t* d_perm, int epochs, int batch, float lr0, float lr1,
+                     float* h_epoch_loss) {
+  if (!m || member < 0 || member >= m->members || n < batch || batch < 64 || batch > m->maxb ||
+      !d_x || !d_y || !d_perm || epochs < 1)
+    return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  const int64_t steps_per_epoch = n / batch, total = steps_per_epoch * epochs;
+  int64_t k = 0;
+  for (int ep = 0; ep < epochs; ++ep) {
+    fill(m->stream, 1, m->loss, 0.f);
+    for (int64_t st = 0; st < steps_per_epoch; ++st, ++k) {
+      // cosine decay from lr0 to lr1
+      float lr = lr1 + 0.5f * (lr0 - lr1) * (1.f + std::cos(3.14159265f * float(k) / float(total)));
+      // epoch ep reads the permutation from a rotating offset
+      int64_t start = (st * batch + int64_t(ep) * 7919 * batch) % (n - batch + 1);
+      hipLaunchKernelGGL(k_gather_batch, dim3((batch + 7) / 8), dim3(32, 8), 0, m->stream, batch,
+                         d_perm, start, d_x, d_y, m->Xb, m->Tb);
+      forward(m, member, batch);
+      backward(m, member, batch);
+      adam(m, member, lr);
+    }
+    float loss = 0.f;
+    if (hipMemcpyAsync(&loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+      return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train: loss readback");
+    if (h_epoch_loss) h_epoch_loss[ep] = loss / float(steps_per_epoch * batch * kOut);
+  }
+  hipError_t e = hipStreamSynchronize(m->stream);
+  return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train");
+}
+
+int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, float* d_y) {
+  if (!m || member < 0 || member >= m->members || rows < 1 || rows > m->maxb || !d_x || !d_y)
+    return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  std::vector<int32_t> ident(rows);
+  for (int i = 0; i < rows; ++i) ident[i] = i;
+  int32_t* perm = nullptr;
+  if (hipMalloc(&perm, sizeof(int32_t) * rows) != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_predict: hipMalloc");
+  hipError_t e = hipMemcpy(perm, ident.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_gather_batch, dim3((rows + 7) / 8), dim3(32, 8), 0, m->stream, rows, perm,
+                     int64_t(0), d_x, static_cast<const float*>(nullptr), m->Xb, m->Tb);
+  forward(m, member, rows);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(d_y, sizeof(float) * kOut, m->Z[kLayers - 1], sizeof(float) * kNL,
+                         sizeof(float) * kOut, rows, hipMemcpyDeviceToDevice, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  (void)hipFree(perm);
+  return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_predict");
+}
+
+}  // extern "C"

@@ -204,6 +204,44 @@ int bnsamd_server_get_stats(bnsamd_server* s, bnsamd_server_stats* out);
 int bnsamd_host_alloc(size_t bytes, void** ptr);
 void bnsamd_host_free(void* ptr);
 
+/* ---- learned form: BF16-MFMA MLP ensemble ------------------------------------
+ * No counterpart in the reference.  A small network per ensemble member maps
+ * 12 features of a point (log nb, log T, Ye and, for nue, anue, nux, log n,
+ * log J, chi) to 15 standardised logarithms of the outputs (eta_0, eta,
+ * kappa_0_a, kappa_a, kappa_s of nue, anue, nux); it is trained on the GPU by
+ * the library's own kernels on data the FP64 path generates, and evaluated by
+ * a fused kernel whose points fall back to the FP64 path when the members
+ * disagree or the inputs leave the training box. */
+typedef struct bnsamd_mlp bnsamd_mlp;
+
+/* feature and output standardisation, and the training box (raw features) */
+typedef struct bnsamd_mlp_norm {
+  float feat_mean[12], feat_std[12];
+  float out_mean[15], out_std[15];
+  float box_lo[12], box_hi[12];
+} bnsamd_mlp_norm;
+
+int bnsamd_mlp_create(int device, int members, int max_batch, bnsamd_mlp** out);
+void bnsamd_mlp_destroy(bnsamd_mlp* m);
+const char* bnsamd_mlp_last_error(const bnsamd_mlp* m);
+/* floats per member in the parameter blob (W_0, b_0, ..., W_4, b_4) */
+int bnsamd_mlp_param_count(void);
+int bnsamd_mlp_set_params(bnsamd_mlp* m, int member, const float* h_params);
+int bnsamd_mlp_get_params(bnsamd_mlp* m, int member, float* h_params);
+int bnsamd_mlp_set_norm(bnsamd_mlp* m, const bnsamd_mlp_norm* norm);
+/* loss and gradient (host blob) of one member on `rows` rows of normalised
+ * features d_x [rows][12] and standardised targets d_y [rows][15] */
+int bnsamd_mlp_gradients(bnsamd_mlp* m, int member, int rows, const float* d_x, const float* d_y,
+                         float* h_grad, float* h_loss);
+/* Adam training of one member: `epochs` passes over n rows in the order
+ * d_perm, batches of `batch` rows, cosine learning rate lr0 -> lr1; the mean
+ * loss of each epoch goes to h_epoch_loss[epochs] (may be NULL) */
+int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, const float* d_y,
+                     const int32_t* d_perm, int epochs, int batch, float lr0, float lr1,
+                     float* h_epoch_loss);
+/* standardised outputs d_y [rows][15] of one member (training-path kernels) */
+int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, float* d_y);
+
 #ifdef __cplusplus
 }
 #endif
