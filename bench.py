@@ -194,6 +194,54 @@ def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
                        "max_concurrent_batches": st["max_concurrent_batches"]}}
 
 
+def surrogate_eval(ctx, dpts, ref, fp64_ms, params, nquad, train_points, epochs, thresholds,
+                   steps=3):
+    """The learned form: an ensemble of 4 MLPs trained on the GPU (own
+    kernels) on train_points seeded points (seed 1000, disjoint from the
+    evaluation grid) with FP64-path targets at order nquad, then evaluated on
+    the workload points with per-point fallback to the FP64 path; accuracy
+    judged against the FP64 outputs ref."""
+    import torch
+    from bns_nurates_amd import surrogate as S
+    from bns_nurates_amd import workloads as W
+    t0 = time.perf_counter()
+    ens = S.train_ensemble(ctx, train_points, nquad, params, members=4, epochs=epochs,
+                           batch=4096, seed=1000, log=log)
+    train_s = time.perf_counter() - t0
+    npts = dpts.shape[1]
+    r = ref.cpu().numpy()
+    rows = S.OUT_ROWS
+    res = {"members": 4, "network": "12 -> 4 x 256 SiLU -> 15, BF16 MFMA, FP32 accumulation",
+           "train_points": int(ens.n_train), "epochs": epochs, "train_wall_s": round(train_s, 1),
+           "final_epoch_loss": [float(f"{h[-1]:.4g}") for h in ens.history],
+           "fp64_kernel_ms": fp64_ms, "thresholds": []}
+    out = torch.empty_like(ref)
+    for thr in thresholds:
+        ctx.set_quadrature(nquad)
+        nfb = ens.m1_opacities(ctx, dpts, out, thr, W.FLAGS_ALL, params)  # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            nfb = ens.m1_opacities(ctx, dpts, out, thr, W.FLAGS_ALL, params)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t1) * 1e3 / steps
+        o = out.cpu().numpy()
+        acc = ~np.all(o == r, axis=0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rel = np.abs(o[rows] - r[rows]) / np.abs(r[rows])
+        per = {W.OUT_NAMES[q]: [float(f"{rel[i, acc].max():.3g}"), float(f"{np.median(rel[i, acc]):.3g}")]
+               for i, q in enumerate(rows)} if acc.any() else {}
+        res["thresholds"].append({
+            "spread_threshold": thr, "fallback_fraction": round(nfb / npts, 4),
+            "ms_per_pass": round(ms, 2), "points_per_s": float(f"{npts / (ms * 1e-3):.4g}"),
+            "speedup_vs_fp64": round(fp64_ms / ms, 3),
+            "accepted_rel_err_max_median": per})
+        log(f"surrogate thr={thr}: fallback {nfb / npts:.1%}, {ms:.1f} ms/pass "
+            f"({fp64_ms / ms:.2f}x FP64)")
+    ens.close()
+    return res
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -232,6 +280,9 @@ def main():
     ap.add_argument("--no-hr98", action="store_true", help="skip the HR98 passes")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision passes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned form")
+    ap.add_argument("--surrogate-train-points", type=int, default=10_000_000)
+    ap.add_argument("--surrogate-epochs", type=int, default=8)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -367,6 +418,14 @@ def main():
             "rel_err_vs_fp64_oracle": acc}
     if not args.no_serving:
         line["serving"] = serving_sweep()
+    if not args.no_surrogate:
+        params = W.params_default_all(W.BREM_BRT06)
+        ctx.set_quadrature(args.nquad)
+        ctx.compute_m1_opacities(dpts, flags=W.FLAGS_ALL, params=params, out=out)
+        ref = out.clone()
+        line["surrogate"] = surrogate_eval(ctx, dpts, ref, b["kernel_ms"], params, args.nquad,
+                                           args.surrogate_train_points, args.surrogate_epochs,
+                                           thresholds=(1e-2, 1e-1))
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(pts, args.nquad, W.params_default_all(W.BREM_BRT06))

@@ -216,6 +216,216 @@ __global__ void k_fill(int64_t n, float* __restrict__ p, float v) {
   if (i < n) p[i] = v;
 }
 
+// ---- inference ---------------------------------------------------------------------
+// BF16 weights of one layer in MFMA fragment order: block (kb, nb) of 32 x 16
+// holds, for lane l, W[32 kb + 8 (l / 16) + j][16 nb + l % 16], j = 0..7.
+__global__ void k_pack(int K, int N, const float* __restrict__ Wf, uint16_t* __restrict__ Wp) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(K) * N) return;
+  int j = int(i & 7), lane = int((i >> 3) & 63);
+  int64_t blk = i >> 9;
+  int NB = N / 16, kb = int(blk / NB), nb = int(blk % NB);
+  int k = 32 * kb + 8 * (lane >> 4) + j, n = 16 * nb + (lane & 15);
+  Wp[i] = f2bf(Wf[int64_t(k) * N + n]);
+}
+
+struct InferArgs {
+  int64_t npts, ld;
+  const double* pts;                 // [21][ld]
+  const bnsamd_mlp_norm* norm;
+  const uint16_t* wp[kMaxMembers];   // packed weights per member
+  const float* w[kMaxMembers];       // FP32 blobs (biases)
+  int members;
+  float threshold;                   // ensemble spread (natural log) that triggers fallback
+  double* out;                       // [20][ld]
+  int32_t* fallback_idx;             // compacted indices of the points to recompute
+  int32_t* fallback_count;
+};
+
+constexpr int kRows = 64, kLds = kHidden + 8;  // points per workgroup, padded row
+
+// A (from LDS, [64][stride]) times the packed weights of one layer, K steps
+// of 32, for this wave's column tiles; result in acc[rt][ct]
+template <int NCT>
+__device__ inline void mma_layer(const uint16_t* __restrict__ A, int stride, int K,
+                                 const uint16_t* __restrict__ Wp, int NB, int nb0, int lane,
+                                 floatx4 (&acc)[4][4]) {
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int kb = 0; kb < K / 32; ++kb) {
+    bf16x8 b[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct)
+      b[ct] = ld8(Wp + ((int64_t(kb) * NB + nb0 + ct) * 64 + lane) * 8);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      bf16x8 a = ld8(A + (rt * 16 + (lane & 15)) * stride + 32 * kb + 8 * (lane >> 4));
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct)
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ct], acc[rt][ct], 0, 0, 0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_infer(InferArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t X[kRows][kK0 + 8];
+  __shared__ __attribute__((aligned(16))) uint16_t Ha[kRows][kLds];
+  __shared__ __attribute__((aligned(16))) uint16_t Hb[kRows][kLds];
+  __shared__ float spread[kRows][kNL];
+  __shared__ float xs[kRows][kFeat];  // standardised features (linear output model)
+  __shared__ int oob[kRows];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t p0 = int64_t(blockIdx.x) * kRows;
+  const bnsamd_mlp_norm& nm = *a.norm;
+
+  // features of this workgroup's points (thread p), normalised, checked
+  // against the training box
+  if (t < kRows) {
+    int64_t p = p0 + t;
+    float f[kFeat];
+    bool in = p < a.npts;
+    if (in) {
+      const double* q = a.pts + p;
+      const int64_t ld = a.ld;
+      f[0] = float(log(q[0 * ld]));
+      f[1] = float(log(q[1 * ld]));
+      f[2] = float(q[2 * ld]);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        f[3 + 3 * s] = float(log(q[(9 + s) * ld]));
+        f[4 + 3 * s] = float(log(q[(13 + s) * ld]));
+        f[5 + 3 * s] = float(q[(17 + s) * ld]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kFeat; ++i) f[i] = nm.feat_mean[i];
+    }
+    bool out_of_box = !in;
+#pragma unroll
+    for (int i = 0; i < kFeat; ++i) {
+      out_of_box = out_of_box || !(f[i] >= nm.box_lo[i] && f[i] <= nm.box_hi[i]);
+      const float x = (f[i] - nm.feat_mean[i]) / nm.feat_std[i];
+      xs[t][i] = out_of_box ? 0.f : x;
+      X[t][i] = f2bf(x);
+    }
+#pragma unroll
+    for (int i = kFeat; i < kK0 + 8; ++i) X[t][i] = 0;
+    oob[t] = out_of_box;
+  }
+  __syncthreads();
+
+  // running ensemble sums of this lane's four outputs (rows 16 wave +
+  // 4 (lane / 16) + v, column lane % 16)
+  float sum[4] = {0.f, 0.f, 0.f, 0.f}, sq[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int e = 0; e < a.members; ++e) {
+    const uint16_t* wp = a.wp[e];
+    const float* wf = a.w[e];
+    const uint16_t* in = &X[0][0];
+    int in_stride = kK0 + 8;
+    for (int l = 0; l < kLayers - 1; ++l) {
+      const int K = l ? kHidden : kK0;
+      floatx4 acc[4][4];
+      mma_layer<4>(in, in_stride, K, wp + kPOff[l], kHidden / 16, 4 * wave, lane, acc);
+      uint16_t* dst = (l & 1) ? &Hb[0][0] : &Ha[0][0];
+      const float* bias = wf + kBOff[l];
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const int col = 64 * wave + 16 * ct + (lane & 15);
+        const float bc = bias[col];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            dst[(rt * 16 + 4 * (lane >> 4) + v) * kLds + col] = f2bf(silu(acc[rt][ct][v] + bc));
+      }
+      __syncthreads();
+      in = dst;
+      in_stride = kLds;
+    }
+    // output layer: wave w takes rows 16 w .. 16 w + 15
+    floatx4 o = floatx4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* wl = wp + kPOff[kLayers - 1];
+    for (int kb = 0; kb < kHidden / 32; ++kb) {
+      bf16x8 b = ld8(wl + (int64_t(kb) * 64 + lane) * 8);
+      bf16x8 x = ld8(in + (16 * wave + (lane & 15)) * kLds + 32 * kb + 8 * (lane >> 4));
+      o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, b, o, 0, 0, 0);
+    }
+    const float bo = wf[kBOff[kLayers - 1] + (lane & 15)];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float y = o[v] + bo;
+      sum[v] += y;
+      sq[v] += y * y;
+    }
+    __syncthreads();  // the next member overwrites Ha / Hb
+  }
+
+  // ensemble mean -> outputs; spread (natural-log units) -> fallback flag
+  const int col = lane & 15;
+  const float inv = 1.f / float(a.members);
+  const float os = col < kOut ? nm.out_std[col] : 0.f, om = col < kOut ? nm.out_mean[col] : 0.f;
+  const float rlo = col < kOut ? nm.res_lo[col] : 0.f, rhi = col < kOut ? nm.res_hi[col] : 0.f;
+  float mean[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    mean[v] = sum[v] * inv;
+    float var = fmaxf(sq[v] * inv - mean[v] * mean[v], 0.f);
+    // a residual outside the training range counts as unbounded spread
+    bool in_range = mean[v] >= rlo && mean[v] <= rhi;
+    spread[16 * wave + 4 * (lane >> 4) + v][col] =
+        col < kOut ? (in_range ? sqrtf(var) * os : __builtin_huge_valf()) : 0.f;
+  }
+  __syncthreads();
+  if (t < kRows) {
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < kOut; ++c) s = fmaxf(s, spread[t][c]);
+    int fb = oob[t] || !(s <= a.threshold);
+    oob[t] = fb;
+    int64_t p = p0 + t;
+    if (fb && p < a.npts) {
+      int slot = atomicAdd(a.fallback_count, 1);
+      a.fallback_idx[slot] = int32_t(p);
+    }
+  }
+  __syncthreads();
+  // outputs of the accepted points: column c = 5 s + q (q: eta_0, eta,
+  // kappa_0_a, kappa_a, kappa_s) goes to field 4 q' + s, anux copying nux
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = 16 * wave + 4 * (lane >> 4) + v;
+    const int64_t p = p0 + r;
+    if (col < kOut && p < a.npts && !oob[r]) {
+      const int s = col / 5, q = col % 5;
+      const int base = q == 0 ? 0 : q == 1 ? 8 : q == 2 ? 4 : q == 3 ? 12 : 16;
+      float lin = nm.lin[0][col];
+#pragma unroll
+      for (int i = 0; i < kFeat; ++i) lin = fmaf(nm.lin[1 + i][col], xs[r][i], lin);
+      const double y = exp(double(lin) + double(mean[v] * os + om));
+      a.out[(base + s) * a.ld + p] = y;
+      if (s == 2) a.out[(base + 3) * a.ld + p] = y;
+    }
+  }
+}
+
+// compact copies of the fallback points and back
+__global__ void k_gather_pts(int32_t n, const int32_t* __restrict__ idx, const double* __restrict__ pts,
+                             int64_t ld, double* __restrict__ cp) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t p = idx[i];
+  for (int f = 0; f < BNSAMD_PT_NFIELDS; ++f) cp[int64_t(f) * n + i] = pts[f * ld + p];
+}
+__global__ void k_scatter_out(int32_t n, const int32_t* __restrict__ idx, const double* __restrict__ co,
+                              double* __restrict__ out, int64_t ld) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t p = idx[i];
+  for (int f = 0; f < BNSAMD_OUT_NFIELDS; ++f) out[f * ld + p] = co[int64_t(f) * n + i];
+}
+
 }  // namespace mlp
 }  // namespace bns
 
@@ -241,6 +451,13 @@ struct bnsamd_mlp {
   uint16_t* dZ[kLayers] = {};
   float* dH = nullptr;
   float* loss = nullptr;
+  // fallback workspace (grown on demand)
+  int64_t fb_cap = 0;
+  int32_t* fb_idx = nullptr;
+  int32_t* fb_count = nullptr;
+  double* fb_pts = nullptr;
+  double* fb_out = nullptr;
+  bool packed_valid = false;
 };
 
 namespace {
@@ -389,6 +606,10 @@ void bnsamd_mlp_destroy(bnsamd_mlp* m) {
   }
   fr(m->dH);
   fr(m->loss);
+  fr(m->fb_idx);
+  fr(m->fb_count);
+  fr(m->fb_pts);
+  fr(m->fb_out);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -410,6 +631,7 @@ int bnsamd_mlp_set_params(bnsamd_mlp* m, int member, const float* h_params) {
       hipMemset(m->v[member], 0, sizeof(float) * kParams) != hipSuccess)
     return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_set_params: copy");
   m->steps[member] = 0;
+  m->packed_valid = false;
   return BNSAMD_OK;
 }
 
@@ -467,6 +689,7 @@ int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, con
       !d_x || !d_y || !d_perm || epochs < 1)
     return BNSAMD_ERR_INVALID;
   (void)hipSetDevice(m->device);
+  m->packed_valid = false;
   const int64_t steps_per_epoch = n / batch, total = steps_per_epoch * epochs;
   int64_t k = 0;
   for (int ep = 0; ep < epochs; ++ep) {
@@ -494,7 +717,7 @@ int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, con
 }
 
 int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, float* d_y) {
-  if (!m || member < 0 || member >= m->members || rows < 1 || rows > m->maxb || !d_x || !d_y)
+  if (!m || member < 0 || member >= m->members || rows < 1 || !d_x || !d_y)
     return BNSAMD_ERR_INVALID;
   (void)hipSetDevice(m->device);
   std::vector<int32_t> ident(rows);
@@ -503,15 +726,101 @@ int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, fl
   if (hipMalloc(&perm, sizeof(int32_t) * rows) != hipSuccess)
     return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_predict: hipMalloc");
   hipError_t e = hipMemcpy(perm, ident.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_gather_batch, dim3((rows + 7) / 8), dim3(32, 8), 0, m->stream, rows, perm,
-                     int64_t(0), d_x, static_cast<const float*>(nullptr), m->Xb, m->Tb);
-  forward(m, member, rows);
-  if (e == hipSuccess)
-    e = hipMemcpy2DAsync(d_y, sizeof(float) * kOut, m->Z[kLayers - 1], sizeof(float) * kNL,
-                         sizeof(float) * kOut, rows, hipMemcpyDeviceToDevice, m->stream);
+  for (int r0 = 0; r0 < rows && e == hipSuccess; r0 += m->maxb) {  // batches of maxb rows
+    const int b = rows - r0 < m->maxb ? rows - r0 : m->maxb;
+    hipLaunchKernelGGL(k_gather_batch, dim3((b + 7) / 8), dim3(32, 8), 0, m->stream, b, perm,
+                       int64_t(r0), d_x, static_cast<const float*>(nullptr), m->Xb, m->Tb);
+    forward(m, member, b);
+    e = hipMemcpy2DAsync(d_y + int64_t(r0) * kOut, sizeof(float) * kOut, m->Z[kLayers - 1],
+                         sizeof(float) * kNL, sizeof(float) * kOut, b, hipMemcpyDeviceToDevice,
+                         m->stream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
   (void)hipFree(perm);
   return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_predict");
+}
+
+int bnsamd_surrogate_m1_opacities(bnsamd_context* ctx, bnsamd_mlp* m, int64_t npts, int64_t ld,
+                                   const double* d_pts, float spread_threshold,
+                                   const bnsamd_flags* flags, const bnsamd_params* params,
+                                   int stim_abs, double* d_out, int64_t* n_fallback) {
+  if (!ctx || !m || npts < 0 || ld < npts || !d_pts || !d_out || !flags || !params ||
+      npts > int64_t(0x7fffffff))
+    return m ? mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_surrogate_m1_opacities: invalid argument")
+             : BNSAMD_ERR_INVALID;
+  if (npts == 0) {
+    if (n_fallback) *n_fallback = 0;
+    return BNSAMD_OK;
+  }
+  (void)hipSetDevice(m->device);
+  hipStream_t s = static_cast<hipStream_t>(bnsamd_get_stream(ctx));
+  // training runs on the MLP's own stream: parameters are final before use
+  if (hipStreamSynchronize(m->stream) != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate_m1_opacities: training stream");
+  if (!m->packed_valid) {
+    for (int e = 0; e < m->members; ++e)
+      for (int l = 0; l < kLayers; ++l) {
+        int64_t n = int64_t(kLayerK[l]) * kLayerN[l];
+        hipLaunchKernelGGL(k_pack, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, kLayerK[l],
+                           kLayerN[l], m->w[e] + kWOff[l], m->wp[e] + kPOff[l]);
+      }
+    m->packed_valid = true;
+  }
+  if (npts > m->fb_cap) {
+    auto fr = [](void* p) {
+      if (p) (void)hipFree(p);
+    };
+    fr(m->fb_idx);
+    fr(m->fb_count);
+    fr(m->fb_pts);
+    fr(m->fb_out);
+    m->fb_idx = nullptr;
+    m->fb_count = nullptr;
+    m->fb_pts = m->fb_out = nullptr;
+    m->fb_cap = 0;
+    if (hipMalloc(&m->fb_idx, sizeof(int32_t) * npts) != hipSuccess ||
+        hipMalloc(&m->fb_count, sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&m->fb_pts, sizeof(double) * BNSAMD_PT_NFIELDS * npts) != hipSuccess ||
+        hipMalloc(&m->fb_out, sizeof(double) * BNSAMD_OUT_NFIELDS * npts) != hipSuccess)
+      return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_surrogate_m1_opacities: workspace");
+    m->fb_cap = npts;
+  }
+  InferArgs a{};
+  a.npts = npts;
+  a.ld = ld;
+  a.pts = d_pts;
+  a.norm = reinterpret_cast<const bnsamd_mlp_norm*>(m->norm);
+  for (int e = 0; e < m->members; ++e) {
+    a.wp[e] = m->wp[e];
+    a.w[e] = m->w[e];
+  }
+  a.members = m->members;
+  a.threshold = spread_threshold;
+  a.out = d_out;
+  a.fallback_idx = m->fb_idx;
+  a.fallback_count = m->fb_count;
+  hipError_t e = hipMemsetAsync(m->fb_count, 0, sizeof(int32_t), s);
+  hipLaunchKernelGGL(k_infer, dim3(unsigned((npts + kRows - 1) / kRows)), dim3(256), 0, s, a);
+  int32_t nfb = 0;
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&nfb, m->fb_count, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate_m1_opacities: infer");
+  if (nfb > 0) {
+    unsigned g = unsigned((nfb + 255) / 256);
+    hipLaunchKernelGGL(k_gather_pts, dim3(g), dim3(256), 0, s, nfb, m->fb_idx, d_pts, ld,
+                       m->fb_pts);
+    int rc = bnsamd_m1_opacities(ctx, nfb, nfb, m->fb_pts, nullptr, flags, params, stim_abs,
+                                 BNSAMD_FP64, m->fb_out);
+    if (rc != BNSAMD_OK)
+      return mlp_fail(m, rc, std::string("fallback: ") + bnsamd_last_error(ctx));
+    hipLaunchKernelGGL(k_scatter_out, dim3(g), dim3(256), 0, s, nfb, m->fb_idx, m->fb_out, d_out,
+                       ld);
+  }
+  if (n_fallback) *n_fallback = nfb;
+  e = hipGetLastError();
+  return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate: fallback");
 }
 
 }  // extern "C"

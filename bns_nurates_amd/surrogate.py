@@ -25,11 +25,14 @@ OUT_ROWS = [r for s in range(3) for r in (0 + s, 8 + s, 4 + s, 12 + s, 16 + s)]
 class MlpNorm(C.Structure):
     _fields_ = [("feat_mean", C.c_float * 12), ("feat_std", C.c_float * 12),
                 ("out_mean", C.c_float * 15), ("out_std", C.c_float * 15),
-                ("box_lo", C.c_float * 12), ("box_hi", C.c_float * 12)]
+                ("box_lo", C.c_float * 12), ("box_hi", C.c_float * 12),
+                ("lin", (C.c_float * 15) * 13),
+                ("res_lo", C.c_float * 15), ("res_hi", C.c_float * 15)]
 
 
 def features(pts: np.ndarray) -> np.ndarray:
-    """Raw features [npts, 12] (float64) of a point block [21, npts]."""
+    """Raw features [npts, 12] (float64) of a point block [21, npts]; a zero
+    density gives -inf (outside any training box)."""
     f = np.empty((pts.shape[1], FEAT))
     f[:, 0] = np.log(pts[W.P_NB])
     f[:, 1] = np.log(pts[W.P_T])
@@ -128,13 +131,19 @@ class Ensemble:
                     "bnsamd_mlp_get_params")
         return p
 
-    def set_norm(self, feat_mean, feat_std, out_mean, out_std, box_lo, box_hi):
+    def set_norm(self, feat_mean, feat_std, out_mean, out_std, box_lo, box_hi, lin, res_lo,
+                 res_hi):
         n = MlpNorm()
         for name, v in (("feat_mean", feat_mean), ("feat_std", feat_std), ("out_mean", out_mean),
-                        ("out_std", out_std), ("box_lo", box_lo), ("box_hi", box_hi)):
+                        ("out_std", out_std), ("box_lo", box_lo), ("box_hi", box_hi),
+                        ("res_lo", res_lo), ("res_hi", res_hi)):
             arr = getattr(n, name)
             for i, x in enumerate(np.asarray(v, np.float64)):
                 arr[i] = float(x)
+        lin = np.asarray(lin, np.float64)
+        for i in range(13):
+            for o in range(15):
+                n.lin[i][o] = float(lin[i, o])
         self._check(self.lib.bnsamd_mlp_set_norm(self.h, C.byref(n)), "bnsamd_mlp_set_norm")
         self.norm = n
 
@@ -157,3 +166,65 @@ class Ensemble:
     def predict(self, member: int, x, y):
         self._check(self.lib.bnsamd_mlp_predict(self.h, member, x.shape[0], _fptr(x), _fptr(y)),
                     "bnsamd_mlp_predict")
+
+    def m1_opacities(self, ctx, pts, out, threshold: float, flags=None, params=None,
+                     stim_abs: int = 1) -> int:
+        """The learned path on a device point block pts [21, >= n] into out
+        [20, >= n] (torch float64 tensors, ctx's quadrature for the fallback);
+        returns the number of points recomputed by the FP64 path."""
+        n = pts.shape[1]
+        pp, ld = api._tensor_ptr(pts, 21, n, "pts")
+        op, ldo = api._tensor_ptr(out, 20, n, "out")
+        if ldo != ld:
+            raise api.BnsAmdError("pts and out must share the leading dimension")
+        ctx._follow_torch_stream()
+        fl, pr = api.as_flags(flags).c(), api.as_params(params).c()
+        nfb = C.c_int64()
+        self._check(self.lib.bnsamd_surrogate_m1_opacities(
+            ctx.h, self.h, n, ld, C.c_void_p(pp), C.c_float(threshold), C.byref(fl), C.byref(pr),
+            int(stim_abs), C.c_void_p(op), C.byref(nfb)), "bnsamd_surrogate_m1_opacities")
+        return int(nfb.value)
+
+
+def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs: int = 8,
+                   batch: int = 4096, lr0: float = 2e-3, lr1: float = 2e-5, seed: int = 1000,
+                   chunk: int = 1 << 21, log=None) -> "Ensemble":
+    """An ensemble trained on npts seeded points (workload distributions,
+    seed `seed`) with targets from the FP64 path at quadrature order nquad;
+    member e starts from its own initialisation and data order."""
+    import torch
+    pts = W.make_points(npts, seed=seed)
+    ctx.set_quadrature(nquad)
+    outs = []
+    for a in range(0, npts, chunk):
+        d = torch.from_numpy(np.ascontiguousarray(pts[:, a:a + chunk])).cuda()
+        outs.append(ctx.compute_m1_opacities(d, flags=W.FLAGS_ALL, params=params).cpu().numpy())
+    out = np.concatenate(outs, axis=1)
+    with np.errstate(divide="ignore"):
+        f = features(pts)
+    t, ok = targets(out)
+    ok &= np.all(np.isfinite(f), axis=1)
+    f, t = f[ok], t[ok]
+    fm, fs = f.mean(0), f.std(0)
+    xn = (f - fm) / fs
+    # linear model of the log outputs in the standardised features; the
+    # network learns the standardised residual
+    A, *_ = np.linalg.lstsq(np.c_[np.ones(len(xn)), xn], t, rcond=None)
+    res = t - np.c_[np.ones(len(xn)), xn] @ A
+    tm, ts = res.mean(0), res.std(0)
+    x = torch.from_numpy(xn.astype(np.float32)).cuda()
+    y = torch.from_numpy(((res - tm) / ts).astype(np.float32)).cuda()
+    ens = Ensemble(ctx.device, members=members, max_batch=batch)
+    ys = (res - tm) / ts
+    ens.set_norm(fm, fs, tm, ts, f.min(0), f.max(0), A, ys.min(0), ys.max(0))
+    ens.lin = A
+    ens.n_train, ens.history = int(ok.sum()), []
+    for e in range(members):
+        ens.set_params(e, init_params(seed + 17 * e))
+        perm = torch.from_numpy(np.random.default_rng(seed + 31 * e).permutation(len(f))
+                                .astype(np.int32)).cuda()
+        h = ens.train(e, x, y, perm, epochs, batch, lr0, lr1)
+        ens.history.append(h)
+        if log:
+            log(f"surrogate member {e}: epoch losses {np.array2string(h, precision=5)}")
+    return ens

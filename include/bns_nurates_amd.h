@@ -214,11 +214,17 @@ void bnsamd_host_free(void* ptr);
  * disagree or the inputs leave the training box. */
 typedef struct bnsamd_mlp bnsamd_mlp;
 
-/* feature and output standardisation, and the training box (raw features) */
+/* feature standardisation x = (f - feat_mean) / feat_std, the training box
+ * (raw features), and the output model: log output o =
+ * lin[0][o] + sum_i lin[1 + i][o] x_i + out_mean[o] + out_std[o] y_o, where
+ * y is the network's output (the network learns the residual of a linear
+ * model in the standardised features) */
 typedef struct bnsamd_mlp_norm {
   float feat_mean[12], feat_std[12];
   float out_mean[15], out_std[15];
   float box_lo[12], box_hi[12];
+  float lin[13][15];
+  float res_lo[15], res_hi[15]; /* range of the standardised residual in training */
 } bnsamd_mlp_norm;
 
 int bnsamd_mlp_create(int device, int members, int max_batch, bnsamd_mlp** out);
@@ -241,6 +247,18 @@ int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, con
                      float* h_epoch_loss);
 /* standardised outputs d_y [rows][15] of one member (training-path kernels) */
 int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, float* d_y);
+/* The learned form of bnsamd_m1_opacities (same point and output blocks, on
+ * the context's stream): every member of the ensemble evaluates each point;
+ * points whose members disagree by more than spread_threshold (standard
+ * deviation of a log output), whose features leave the training box, or
+ * whose predicted residual leaves the range seen in training are
+ * recomputed by the FP64 path with the context's quadrature (which must be
+ * the one the targets were made with), the others take the ensemble mean.
+ * The number of recomputed points goes to *n_fallback.  Synchronous. */
+int bnsamd_surrogate_m1_opacities(bnsamd_context* ctx, bnsamd_mlp* m, int64_t npts, int64_t ld,
+                                  const double* d_pts, float spread_threshold,
+                                  const bnsamd_flags* flags, const bnsamd_params* params,
+                                  int stim_abs, double* d_out, int64_t* n_fallback);
 
 #ifdef __cplusplus
 }
