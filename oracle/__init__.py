@@ -7,6 +7,8 @@ the cpu_baseline leg of bench.py -- as the checker, never as the product.
 * ``liborc_count.so`` -- the same code over a FLOP-counting real type
 * ``liborc_bound.so`` -- the same code over a magnitude-propagating type: a
   running error bound m per output (see oracle/error_bound.cpp)
+* ``liborc_fast.so``  -- the same code built for speed (-O3, AVX2/FMA): the
+  timed CPU baseline of bench.py (not bit-exact: contraction is allowed)
 * ``_ref/libbnsref.so`` -- the genuine reference compiled from its headers
   (n <= 10 only: BS_N_MAX = 20 in bns_nurates.hpp:68)
 
@@ -25,6 +27,7 @@ _LIBS = {
     "orc": os.path.join(HERE, "liborc.so"),
     "count": os.path.join(HERE, "liborc_count.so"),
     "bound": os.path.join(HERE, "liborc_bound.so"),
+    "fast": os.path.join(HERE, "liborc_fast.so"),
     "ref": os.path.join(HERE, "_ref", "libbnsref.so"),
 }
 _cache = {}
