@@ -261,7 +261,11 @@ const char* bnsamd_last_error(const bnsamd_context* c) { return c ? c->err.c_str
 int bnsamd_set_stream(bnsamd_context* c, void* s) {
   if (check_ctx(c)) return BNSAMD_ERR_INVALID;
   std::lock_guard<std::mutex> g(c->mu);
-  c->stream = static_cast<hipStream_t>(s);  // NULL: HIP's default (null) stream
+  hipStream_t ns = static_cast<hipStream_t>(s);  // NULL: HIP's default (null) stream
+  // the staging buffer of the host entry is reused in stream order: finish
+  // what the previous stream still has queued on it
+  if (ns != c->stream && c->stage) (void)hipStreamSynchronize(c->stream);
+  c->stream = ns;
   return BNSAMD_OK;
 }
 
