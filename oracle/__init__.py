@@ -10,7 +10,7 @@ the cpu_baseline leg of bench.py -- as the checker, never as the product.
 * ``liborc_fast.so``  -- the same code built for speed (-O3, AVX2/FMA): the
   timed CPU baseline of bench.py (not bit-exact: contraction is allowed)
 * ``_ref/libbnsref.so`` -- the genuine reference compiled from its headers
-  (n <= 10 only: BS_N_MAX = 20 in bns_nurates.hpp:68)
+  (gray n <= 10, spectral n <= 20: BS_N_MAX = 20 in bns_nurates.hpp:68)
 
 Both ``orc_*`` and ``ref_*`` entry points share one signature per function,
 so every wrapper below takes the library and a prefix.
@@ -87,6 +87,9 @@ def load(kind: str = "orc"):
         "total_nuf": ([_i64, _dp, _dp, C.c_int, _dp], None),
         "m1_opacities": ([_i64, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_void_p, _ip,
                           _ip, C.c_int, _dp, C.c_int], C.c_int),
+        "iso_leg0": ([_i64, _dp, _dp, _ip, _dp], None),
+        "spectral_opacities": ([_i64, C.c_int, _dp, _dp, _dp, C.c_void_p, _i64, _dp, _i64,
+                                _ip, _ip, C.c_int, _dp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, f"{p}_{name}")
@@ -211,6 +214,65 @@ def m1_opacities(pts, nquad, flags, params, stim=1, distr=None, kind="orc", nthr
     return out
 
 
+def _nu_arg(nu, npts):
+    """(nu array, nnu, nu_ld): a 1-D grid is shared by every point, a 2-D
+    [nnu, npts] array gives every point its own energies."""
+    nu = _f64(nu)
+    if nu.ndim == 1:
+        return nu, nu.shape[0], 0
+    if nu.ndim != 2 or nu.shape[1] != npts:
+        raise ValueError("nu must be [nnu] or [nnu, npts]")
+    return nu, nu.shape[0], npts
+
+
+def spectral_opacities(pts, nu, nquad, flags, params, stim=0, distr=None, kind="orc",
+                       nthreads=0):
+    """Spectral opacities [16, nnu, npts] (j, kappa, j_s, kappa_s per
+    species) for the point block pts [21, npts] at the energies nu."""
+    pts = _f64(pts)
+    npts = pts.shape[1]
+    nu, nnu, nu_ld = _nu_arg(nu, npts)
+    x, w = gauss_legendre(nquad, kind=kind)
+    out = np.zeros((16, nnu, npts))
+    dptr = None
+    if distr is not None:
+        distr = _f64(distr)
+        dptr = distr.ctypes.data_as(C.c_void_p)
+    rc = _fn(kind, "spectral_opacities")(npts, nquad, x, w, pts, dptr, nnu, nu, nu_ld,
+                                         np.ascontiguousarray(flags, np.int32),
+                                         np.ascontiguousarray(params, np.int32), int(stim),
+                                         out, int(nthreads))
+    if rc != 0:
+        raise ValueError(f"{kind}_spectral_opacities returned {rc}")
+    return out
+
+
+def iso_leg0(om, eos9, params, kind="orc"):
+    om = _f64(om)
+    out = np.zeros_like(om)
+    _fn(kind, "iso_leg0")(om.size, om, _f64(eos9), np.ascontiguousarray(params, np.int32), out)
+    return out
+
+
+def count_spectral_opacities(pts, nu, nquad, flags, params, stim=0):
+    """Algorithmic operation counts of the spectral path over pts x nu (a
+    shared energy grid): dict as count_m1_opacities, and the outputs."""
+    lib, _ = load("count")
+    pts = _f64(pts)
+    npts = pts.shape[1]
+    nu = _f64(nu)
+    x, w = gauss_legendre(nquad)
+    out = np.zeros((16, nu.shape[0], npts))
+    counts = np.zeros(9, dtype=np.int64)
+    f = lib.orc_count_spectral_opacities
+    f.argtypes = [_i64, C.c_int, _dp, _dp, _dp, _i64, _dp, _ip, _ip, C.c_int, _dp, _i64p]
+    f.restype = C.c_int
+    f(npts, nquad, x, w, pts, nu.shape[0], nu, np.ascontiguousarray(flags, np.int32),
+      np.ascontiguousarray(params, np.int32), int(stim), out, counts)
+    names = ["add", "mul", "div", "sqrt", "exp", "log", "pow", "atan", "cbrt"]
+    return dict(zip(names, (int(c) for c in counts))), out
+
+
 def count_m1_opacities(pts, nquad, flags, params, stim=1):
     """Algorithmic operation counts of the gray path over pts:
     dict(add, mul, div, sqrt, exp, log, pow, atan, cbrt) and the outputs."""
@@ -250,6 +312,28 @@ def bound_m1_opacities(pts, nquad, flags, params, stim=1, distr=None, nthreads=0
                                np.ascontiguousarray(flags, np.int32),
                                np.ascontiguousarray(params, np.int32), int(stim), out, mag,
                                int(nthreads))
+    return out, mag
+
+
+def bound_spectral_opacities(pts, nu, nquad, flags, params, stim=0, distr=None, nthreads=0):
+    """Spectral outputs [16, nnu, npts] and their running error magnitudes."""
+    lib, _ = load("bound")
+    pts = _f64(pts)
+    npts = pts.shape[1]
+    nu, nnu, nu_ld = _nu_arg(nu, npts)
+    x, w = gauss_legendre(nquad)
+    out = np.zeros((16, nnu, npts))
+    mag = np.zeros((16, nnu, npts))
+    dptr = None
+    if distr is not None:
+        distr = _f64(distr)
+        dptr = distr.ctypes.data_as(C.c_void_p)
+    f = lib.orc_bound_spectral_opacities
+    f.argtypes = [_i64, C.c_int, _dp, _dp, _dp, C.c_void_p, _i64, _dp, _i64, _ip, _ip, C.c_int,
+                  _dp, _dp, C.c_int]
+    f.restype = C.c_int
+    f(npts, nquad, x, w, pts, dptr, nnu, nu, nu_ld, np.ascontiguousarray(flags, np.int32),
+      np.ascontiguousarray(params, np.int32), int(stim), out, mag, int(nthreads))
     return out, mag
 
 

@@ -98,4 +98,45 @@ int orc_count_m1_opacities(int64_t np, int n1, const double* x1, const double* w
   return 0;
 }
 
+// The spectral path over np points x nnu energies (arguments of
+// orc_spectral_opacities, distributions reconstructed from M1); counts as
+// for orc_count_m1_opacities.
+int orc_count_spectral_opacities(int64_t np, int n, const double* x, const double* w,
+                                 const double* pts, int64_t nnu, const double* nu,
+                                 const int* flags, const int* params, int stim, double* out,
+                                 int64_t* counts) {
+  g_t = Tally{};
+  Quad q{n, x, w};
+  Flags fl{flags[0], flags[1], flags[2], flags[3], flags[4]};
+  Params pr{params[0], params[1], params[2], params[3], params[4], params[5], params[6], params[7]};
+  for (int64_t i = 0; i < np; ++i) {
+    Eos<Cnt> e{pts[P_NB * np + i], pts[P_T * np + i], pts[P_YP * np + i],
+               pts[P_YN * np + i], pts[P_MUP * np + i], pts[P_MUN * np + i],
+               pts[P_MUE * np + i], pts[P_DU * np + i], pts[P_DMEFF * np + i]};
+    M1<Cnt> m;
+    for (int s = 0; s < 4; ++s) {
+      m.n[s] = pts[(P_N0 + s) * np + i];
+      m.J[s] = pts[(P_J0 + s) * np + i];
+      m.chi[s] = pts[(P_CHI0 + s) * np + i];
+    }
+    Distr<Cnt> d;
+    distr_from_m1(m, d);
+    for (int64_t k = 0; k < nnu; ++k) {
+      Cnt o[16];
+      spectral_point(q, e, d, fl, pr, Cnt(nu[k]), stim, o);
+      for (int f = 0; f < 16; ++f) out[(f * nnu + k) * np + i] = o[f].v;
+    }
+  }
+  counts[0] = g_t.add;
+  counts[1] = g_t.mul;
+  counts[2] = g_t.div;
+  counts[3] = g_t.sqrt;
+  counts[4] = g_t.exp;
+  counts[5] = g_t.log;
+  counts[6] = g_t.pow;
+  counts[7] = g_t.atan;
+  counts[8] = g_t.cbrt;
+  return 0;
+}
+
 }  // extern "C"

@@ -202,6 +202,14 @@ void orc_iso_scatt(int64_t n, const double* om, const double* eos9, const int* p
   for (int64_t i = 0; i < n; ++i) out[i] = iso_scatt_total(om[i], p, e);
 }
 
+// IsoScattLegCoeff, l = 0 (formulas_beta_iso.hpp:524-560)
+void orc_iso_leg0(int64_t n, const double* om, const double* eos9, const int* params,
+                  double* out) {
+  Eos<double> e{eos9[0], eos9[1], eos9[2], eos9[3], eos9[4], eos9[5], eos9[6], eos9[7], eos9[8]};
+  Params p = params_from(params);
+  for (int64_t i = 0; i < n; ++i) out[i] = iso_leg0(om[i], p, e);
+}
+
 // CalculateDistrParamsFromM1 (distribution.hpp:323-333) over the SoA point
 // block; distr receives the 28-field block.
 void orc_distr_from_m1(int64_t np, const double* pts, double* distr) {
@@ -270,6 +278,35 @@ int orc_m1_opacities(int64_t np, int n1, const double* x1, const double* w1, int
       double o[20];
       m1_opacities_point(q1, q2, e, m, d, fl, pr, stim, M, o);
       for (int f = 0; f < 20; ++f) out[f * np + i] = o[f];
+    }
+  }
+  return 0;
+}
+
+// ComputeSpectralOpacitiesStimulatedAbs (stim = 1) / ...NotStimulatedAbs
+// (stim = 0), m1_opacities.hpp:1505-1660, for np points x nnu energies with
+// the 1D quadrature (n, x, w).  Energies: nu[e] for every point when
+// nu_ld == 0, else nu[e * nu_ld + i].  out[(f * nnu + e) * np + i], f in
+// S_J.. S_KAPPAS + 3.  distr NULL: reconstructed from the M1 moments.
+int orc_spectral_opacities(int64_t np, int n, const double* x, const double* w,
+                           const double* pts, const double* distr, int64_t nnu,
+                           const double* nu, int64_t nu_ld, const int* flags,
+                           const int* params, int stim, double* out, int nthreads) {
+  if (np < 0 || nnu < 0 || n < 1) return 1;
+  Quad q{n, x, w};
+  Flags fl = flags_from(flags);
+  Params pr = params_from(params);
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4) collapse(2)
+  for (int64_t i = 0; i < np; ++i) {
+    for (int64_t e = 0; e < nnu; ++e) {
+      Eos<double> eos = eos_at(pts, np, i);
+      Distr<double> d;
+      if (distr) distr_load(distr, np, i, d);
+      else distr_from_m1(m1_at(pts, np, i), d);
+      double o[16];
+      spectral_point(q, eos, d, fl, pr, nu[nu_ld ? e * nu_ld + i : e], stim, o);
+      for (int f = 0; f < 16; ++f) out[(f * nnu + e) * np + i] = o[f];
     }
   }
   return 0;

@@ -36,6 +36,12 @@
 //   gray assembly      reference end-to-end vectors (n=4..10), Kirchhoff
 //                      invariant with equilibrium distributions, flag
 //                      additivity
+//   spectral           reference end-to-end vectors (the six thermodynamic
+//                      points of the reference's spectral test, seeded
+//                      points, all flag/parameter variants), Kirchhoff per
+//                      energy at equilibrium, the gray pair/brem/beta sums
+//                      recovered from spectral values at the gray nodes,
+//                      flag additivity, IsoScattLegCoeff closed form
 #ifndef BNS_NURATES_AMD_ORACLE_M1_ORACLE_HPP_
 #define BNS_NURATES_AMD_ORACLE_M1_ORACLE_HPP_
 
@@ -144,6 +150,8 @@ enum {
   O_ETA0 = 0, O_KAPPA0A = 4, O_ETA = 8, O_KAPPAA = 12, O_KAPPAS = 16,
   O_NFIELDS = 20
 };
+// SpectralOpacities (bns_nurates.hpp:516-522), one value per species
+enum { S_J = 0, S_KAPPA = 4, S_JS = 8, S_KAPPAS = 12, S_NFIELDS = 16 };
 
 struct Flags {  // OpacityFlags (bns_nurates.hpp:418-426)
   int use_abs_em, use_pair, use_brem, use_inelastic_scatt, use_iso;
@@ -587,6 +595,19 @@ inline R iso_scatt_total(R omega, const Params& p, const Eos<R>& eos) {
     tot[reac - 1] = etaNN * (leg0 - leg1 / R(3.));
   }
   return tot[0] + tot[1];
+}
+
+// IsoScattLegCoeff, l = 0 (formulas_beta_iso.hpp:524-560)
+template <class R>
+inline R iso_leg0(R omega, const Params& p, const Eos<R>& eos) {
+  R eta_pp = eta_nn_sc(eos.nb, eos.T, eos.yp);
+  R eta_nn = eta_nn_sc(eos.nb, eos.T, eos.yn);
+  R R0p = R(1.), R1p = R(1.), R0n = R(1.), R1n = R(1.);
+  if (p.use_WM_sc) {
+    wm_scatt(omega, 1, R0p, R1p);
+    wm_scatt(omega, 2, R0n, R1n);
+  }
+  return eta_pp * R(k::IsoC0p) * R0p + eta_nn * R(k::IsoC0n) * R0n;
 }
 
 // ---------------------------------------------------------------------------
@@ -1436,6 +1457,169 @@ inline void m1_opacities_point(const Quad& q1, const Quad& q2, const Eos<R>& eos
     if (J[s] > R(k::ThresholdJ)) {
       out[O_KAPPAA + s] = F2 / (c * J[s]) * (e2d[4 + s] + enp[4 + s]);
       out[O_KAPPAS + s] = F / (c * J[s]) * iso[s];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral opacities at one neutrino energy nu
+// (ComputeSpectralOpacitiesNotStimulatedAbs / ...StimulatedAbs,
+// m1_opacities.hpp:1505-1660).
+// ---------------------------------------------------------------------------
+// Per-point NES/NPS scalars (the F_k(+-eta) that NESKernels / NPSKernels
+// evaluate at every call, kernel_nes.hpp:139-340)
+template <class R>
+inline void neps_point(const Eos<R>& eos, NepsPoint<R>& P) {
+  P.T = eos.T;
+  P.eta[0] = eos.mu_e / eos.T;
+  P.eta[1] = -eos.mu_e / eos.T;
+  for (int lep = 0; lep < 2; ++lep) {
+    R b[6];
+    fdi_bundle(0, P.eta[lep], b);
+    for (int kk = 0; kk < 6; ++kk) P.Fe[lep][kk] = b[kk];
+    P.fd0[lep] = fermi_distr(R(0.), R(1.), P.eta[lep]);
+  }
+}
+
+// SpectralIntegrand (m1_opacities.hpp:1316-1501) at the partner energy nub:
+// f[s] = nub^2 x production, f[4 + s] = nub^2 x annihilation, for the
+// reactions switched on in fl (kirchoff_flag is false, :1451-1464).
+template <class R>
+inline void spectral_integrand(R nu, R nub, const Eos<R>& eos, const Distr<R>& d,
+                               const Flags& fl, const Params& pr, const NepsPoint<R>& P,
+                               R f[8]) {
+  const R T = eos.T;
+  R gb[4];
+  for (int s = 0; s < 4; ++s) gb[s] = total_nuf(nub, d, s);
+  KOut<R> pk, bk, ik;
+  for (int s = 0; s < 4; ++s) {
+    pk.em[s] = pk.abs[s] = bk.em[s] = bk.abs[s] = ik.em[s] = ik.abs[s] = R(0.);
+  }
+  if (fl.use_pair) {
+    // PairKernels with cos_theta = 1, lmax = 0 (kernel_pair.hpp:165-198)
+    const R eta = eos.mu_e / T;
+    const R y = nu / T, z = nub / T;
+    R F[6][7];
+    for (int kk = 0; kk < 6; ++kk)
+      for (int m = 0; m < 7; ++m) F[kk][m] = R(0.);
+    if (!(eta - y - z > R(200.))) {
+      R b[6];
+      const R args[4] = {eta - y, eta - z, eta + y, eta + z};
+      const int slot[4] = {A_EMY, A_EMZ, A_EPY, A_EPZ};
+      for (int m = 0; m < 4; ++m) {
+        fdi_bundle(1, args[m], b);
+        for (int kk = 1; kk <= 5; ++kk) F[kk][slot[m]] = b[kk];
+      }
+      fdi_bundle(3, eta, b);
+      for (int kk = 3; kk <= 5; ++kk) F[kk][A_E] = b[kk];
+      fdi_bundle(3, eta - y - z, b);
+      for (int kk = 3; kk <= 5; ++kk) F[kk][A_EMYZ] = b[kk];
+      fdi_bundle(3, eta + y + z, b);
+      for (int kk = 3; kk <= 5; ++kk) F[kk][A_EPYZ] = b[kk];
+    }
+    pair_kernel_fwd(nu, nub, T, eta, y, z, F, pk);
+  }
+  if (fl.use_brem) {
+    R se = R(0.), sa = R(0.);
+    if (pr.brem_impl == BREM_BRT06) brem_brt06(nu, nub, eos, se, sa);
+    else if (pr.brem_impl == BREM_HR98) brem_hr98(nu, nub, eos, pr.use_NN_medium_corr != 0, se, sa);
+    // BREM_GP19: BremKernelAbsGP19 is zero in the reference's table-less
+    // configuration (kernel_brem_GP19.hpp:483-493)
+    for (int s = 0; s < 4; ++s) {
+      bk.em[s] = se;
+      bk.abs[s] = sa;
+    }
+  }
+  if (fl.use_inelastic_scatt) {
+    R G[2][6][3];
+    neps_fdi_table(nu / T, nub / T, P, G);
+    inel_kernels(nu, nub, P, G, false, ik);
+  }
+  R bf[4];
+  for (int s = 0; s < 4; ++s) bf[s] = pr.neglect_blocking ? R(1.) : R(1.) - gb[s];
+  R pro[4], ann[4];
+  for (int s = 0; s < 4; ++s) pro[s] = (pk.em[s] + bk.em[s]) * bf[partner(s)];
+  for (int s = 0; s < 4; ++s) pro[s] += ik.em[s] * gb[s];
+  for (int s = 0; s < 4; ++s) ann[s] = (pk.abs[s] + bk.abs[s]) * gb[partner(s)];
+  for (int s = 0; s < 4; ++s) ann[s] += ik.abs[s] * bf[s];
+  const R nb2 = nub * nub;
+  for (int s = 0; s < 4; ++s) {
+    f[s] = nb2 * pro[s];
+    f[4 + s] = nb2 * ann[s];
+  }
+}
+
+// GaussLegendreIntegrate1D (integration.hpp:511-546) of the eight spectral
+// integrands over (0, inf) with scale t: nodes t x_i and t / x_i, the second
+// half weighted by 1 / x_i^2, each half summed in node order
+// (DoIntegration, integration.hpp:97-108).
+template <class R>
+inline void spectral_integrate(const Quad& q, R t, R nu, const Eos<R>& eos, const Distr<R>& d,
+                               const Flags& fl, const Params& pr, const NepsPoint<R>& P,
+                               R res[8]) {
+  const int n = q.n;
+  std::vector<R> f1(size_t(8) * n), f2(size_t(8) * n);
+  for (int i = 0; i < n; ++i) {
+    const R xi = R(q.x[i]);
+    R v[8];
+    spectral_integrand(nu, t * xi, eos, d, fl, pr, P, v);
+    for (int kk = 0; kk < 8; ++kk) f1[size_t(kk) * n + i] = v[kk];
+    spectral_integrand(nu, t / xi, eos, d, fl, pr, P, v);
+    for (int kk = 0; kk < 8; ++kk) f2[size_t(kk) * n + i] = v[kk] / (xi * xi);
+  }
+  for (int kk = 0; kk < 8; ++kk) {
+    R s1 = R(0.), s2 = R(0.);
+    for (int i = 0; i < n; ++i) s1 += R(q.w[i]) * f1[size_t(kk) * n + i];
+    for (int i = 0; i < n; ++i) s2 += R(q.w[i]) * f2[size_t(kk) * n + i];
+    res[kk] = t * (s1 + s2);
+  }
+}
+
+// ComputeSpectralOpacitiesNotStimulatedAbs (m1_opacities.hpp:1505-1641) and,
+// with stim = 1, ComputeSpectralOpacitiesStimulatedAbs (:1644-1660).
+// out[S_J + s], out[S_KAPPA + s], out[S_JS + s], out[S_KAPPAS + s].
+template <class R>
+inline void spectral_point(const Quad& q, const Eos<R>& eos, const Distr<R>& d,
+                           const Flags& fl, const Params& pr, R nu, int stim, R out[16]) {
+  NepsPoint<R> P;
+  neps_point(eos, P);
+  R g_nu[4];
+  for (int s = 0; s < 4; ++s) g_nu[s] = total_nuf(nu, d, s);
+  // pair + bremsstrahlung with the NEPS flag cleared, scale 2.182 T
+  // (:1519-1560); NEPS alone, scale nu (:1562-1570)
+  const R s_pair = R(0.5 * 4.364) * eos.T;
+  Flags f1 = fl;
+  f1.use_inelastic_scatt = 0;
+  R Ip[8], In[8];
+  spectral_integrate(q, s_pair, nu, eos, d, f1, pr, P, Ip);
+  for (int kk = 0; kk < 8; ++kk) In[kk] = R(0.);
+  if (fl.use_inelastic_scatt == 1) {
+    Flags f2{0, 0, 0, 1, 0};
+    spectral_integrate(q, nu, nu, eos, d, f2, pr, P, In);
+  }
+  R em[4], ab[4];
+  for (int s = 0; s < 4; ++s) em[s] = ab[s] = R(0.);
+  if (fl.use_abs_em) {
+    R e2[2], a2[2];
+    abs_opacity(nu, pr, eos, e2, a2);
+    for (int s = 0; s < 2; ++s) {
+      em[s] = e2[s];
+      ab[s] = a2[s];
+    }
+  }
+  R iso = R(0.);
+  if (fl.use_iso) iso = iso_leg0(nu, pr, eos);
+  const R F = R(k::FourPiHc3), c = R(k::Clight), four_pi = R(4. * k::Pi);
+  for (int s = 0; s < 4; ++s) {
+    out[S_J + s] = em[s] + F * (Ip[s] + In[s]);
+    out[S_KAPPA + s] = (ab[s] + F * (Ip[4 + s] + In[4 + s])) / c;
+    out[S_JS + s] = four_pi * (nu * nu) * g_nu[s] * iso;
+    out[S_KAPPAS + s] = four_pi * (nu * nu) * (R(1.) - g_nu[s]) * iso / c;
+  }
+  if (stim == 1) {
+    for (int s = 0; s < 4; ++s) {
+      out[S_KAPPA + s] = out[S_J + s] / c + out[S_KAPPA + s];
+      out[S_KAPPAS + s] = out[S_JS + s] / c + out[S_KAPPAS + s];
     }
   }
 }

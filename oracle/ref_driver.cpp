@@ -13,8 +13,9 @@
 // recipe selects it with -D__SYCL_DEVICE_ONLY__.  GP19 is not on the path
 // under test; nothing else in the reference reads that macro.
 //
-// BS_N_MAX is 20 in the reference (bns_nurates.hpp:68), so these entry points
-// accept quadrature orders n <= 10 only and return 2 otherwise.
+// BS_N_MAX is 20 in the reference (bns_nurates.hpp:68), so the gray entry
+// point accepts quadrature orders n <= 10 and the spectral one n <= 20; both
+// return 2 otherwise.
 #include <cstdint>
 #include <cstring>
 
@@ -210,6 +211,13 @@ void ref_iso_scatt(int64_t n, const double* om, const double* eos9, const int* p
   for (int64_t i = 0; i < n; ++i) out[i] = IsoScattTotal(om[i], &p, &e);
 }
 
+void ref_iso_leg0(int64_t n, const double* om, const double* eos9, const int* params,
+                  double* out) {
+  MyEOSParams e = eos_at(eos9, 1, 0);
+  OpacityParams p = params_from(params);
+  for (int64_t i = 0; i < n; ++i) out[i] = IsoScattLegCoeff(om[i], &p, &e, 0);
+}
+
 void ref_distr_from_m1(int64_t np, const double* pts, double* distr) {
   for (int64_t i = 0; i < np; ++i) {
     M1Quantities m;
@@ -298,6 +306,54 @@ int ref_m1_opacities(int64_t np, int n1, const double* x1, const double* w1, int
       out[(8 + s) * np + i] = o.eta[s];
       out[(12 + s) * np + i] = o.kappa_a[s];
       out[(16 + s) * np + i] = o.kappa_s[s];
+    }
+  }
+  return 0;
+}
+
+// ComputeSpectralOpacities{Stimulated,NotStimulated}Abs over np points x
+// nnu energies; arguments and output layout of orc_spectral_opacities.
+int ref_spectral_opacities(int64_t np, int n, const double* x, const double* w,
+                           const double* pts, const double* distr, int64_t nnu,
+                           const double* nu, int64_t nu_ld, const int* flags,
+                           const int* params, int stim, double* out, int nthreads) {
+  if (n > BS_N_MAX || n < 1) return 2;
+  MyQuadrature q = quadrature_default;
+  q.nx = n;
+  for (int i = 0; i < n; ++i) {
+    q.points[i] = x[i];
+    q.w[i] = w[i];
+  }
+  (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 4) collapse(2) if (nthreads != 1)
+  for (int64_t i = 0; i < np; ++i) {
+    for (int64_t e = 0; e < nnu; ++e) {
+      GreyOpacityParams g;
+      std::memset(&g, 0, sizeof(g));
+      g.opacity_pars = params_from(params);
+      g.eos_pars = eos_at(pts, np, i);
+      for (int s = 0; s < 4; ++s) {
+        g.m1_pars.n[s] = pts[(9 + s) * np + i];
+        g.m1_pars.J[s] = pts[(13 + s) * np + i];
+        g.m1_pars.chi[s] = pts[(17 + s) * np + i];
+      }
+      if (distr) distr_load(distr, np, i, g.distr_pars);
+      else g.distr_pars = CalculateDistrParamsFromM1(&g.m1_pars, &g.eos_pars);
+      g.opacity_flags.use_abs_em = flags[0];
+      g.opacity_flags.use_pair = flags[1];
+      g.opacity_flags.use_brem = flags[2];
+      g.opacity_flags.use_inelastic_scatt = flags[3];
+      g.opacity_flags.use_iso = flags[4];
+      MyQuadrature lq = q;
+      const double nue = nu[nu_ld ? e * nu_ld + i : e];
+      SpectralOpacities o = stim ? ComputeSpectralOpacitiesStimulatedAbs(nue, &lq, &g)
+                                 : ComputeSpectralOpacitiesNotStimulatedAbs(nue, &lq, &g);
+      for (int s = 0; s < 4; ++s) {
+        out[((0 + s) * nnu + e) * np + i] = o.j[s];
+        out[((4 + s) * nnu + e) * np + i] = o.kappa[s];
+        out[((8 + s) * nnu + e) * np + i] = o.j_s[s];
+        out[((12 + s) * nnu + e) * np + i] = o.kappa_s[s];
+      }
     }
   }
   return 0;

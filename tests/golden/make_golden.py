@@ -5,7 +5,8 @@ oracle/Makefile into oracle/_ref/libbnsref.so, on seeded and hand-made inputs
 and stores inputs + outputs as .npz (numeric arrays only, no pickles).  Run
 it where the reference tree exists:
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py spectral                     # one group
 
 The fixtures are data: they hold no reference source.
 """
@@ -64,10 +65,94 @@ def edge_points() -> np.ndarray:
     return np.ascontiguousarray(np.array(rows, dtype=np.float64).T)
 
 
-def main():
+def spectral_test_points() -> np.ndarray:
+    """The six thermodynamic points of the reference's spectral test
+    (tests/tests.hpp:893-968) as a point block; the M1 fields are left at 0
+    (the test uses equilibrium distributions)."""
+    rows = [
+        # nb [nm^-3], T, Yp, Yn, mu_p, mu_n, mu_e, dU, mn* - mp*
+        (4.208366627847035e+38 * 1.0E-21, 12.406403541564941, 0.07158458, 0.92841542,
+         1011.01797737, 1221.59013681, 187.1814489, 18.92714728, 280.16495513 - 278.87162217),
+        (6.051432235073535e+37 * 1.0E-21, 16.887954711914062, 0.04352445, 0.91736565,
+         843.50691102, 943.76817391, 82.3367694, 35.09393792, 733.97246814 - 732.67912955),
+        (6.0146859596912e+36 * 1.0E-21, 8.919591903686523, 0.02018168, 0.88856839,
+         889.20482938, 930.55123963, 36.56143955, 4.98247735, 915.90721218 - 914.61387308),
+        (6.128721244299242e+35 * 1.0E-21, 6.698440074920654, 0.08023092, 0.8442462,
+         903.09559414, 920.74113958, 19.49956921, 0.46637764, 937.06605368 - 935.77271381),
+        (6.07728949804485e+34 * 1.0E-21, 3.586622953414917, 0.11611014, 0.84457226,
+         916.18996071, 924.65692374, 8.98668385, 0.0454689, 939.30610577 - 938.01276696),
+        (6.192921723010272e+33 * 1.0E-21, 2.144158124923706, 0.17166997, 0.79639333,
+         922.72216948, 927.30360027, 4.20812739, 0.0007919, 939.56535 - 938.27201),
+    ]
+    pts = np.zeros((21, len(rows)))
+    pts[:9] = np.array(rows, dtype=np.float64).T
+    pts[17:21] = 1.0 / 3.0
+    return np.ascontiguousarray(pts)
+
+
+def spectral_fixtures():
+    """Spectral opacities (m1_opacities.hpp:1505-1660) end to end."""
+    # the reference's spectral test: 6 points x 100 energies 10^(-1 + 4 i / 100),
+    # equilibrium distributions, all reactions, opacity_params_default_none
+    # (HR98), n = 10, no stimulated absorption (tests/tests.hpp:861-1261)
+    tp = spectral_test_points()
+    deq = O.distr_equilibrium(tp, kind="ref")
+    tp = O.m1_densities_eq(tp, deq, kind="ref")
+    ebins = np.array([10.0 ** (-1.0 + i * (4.0 / 100.0)) for i in range(100)])
+    prm_none = W.params_default_none(W.BREM_HR98)
+    out = O.spectral_opacities(tp, ebins, 10, W.FLAGS_ALL, prm_none, stim=0, distr=deq,
+                               kind="ref")
+    save("spectral_test6_n10_hr98.npz", pts=tp, distr=deq, nu=ebins, out=out, nquad=10,
+         flags=W.FLAGS_ALL, params=prm_none, stim=0)
+
+    # seeded + edge points, M1-reconstructed distributions, reaction and
+    # parameter variants, both absorption forms, n = 4..20
+    p0 = W.make_points(96, seed=21)
+    mix = np.concatenate([edge_points(), p0], axis=1)
+    nu = np.exp(np.linspace(np.log(0.2), np.log(600.0), 12))
+    variants = {}
+    cases = [
+        ("brt06_all_n10_stim", 10, W.FLAGS_ALL, W.params_default_all(W.BREM_BRT06), 1),
+        ("brt06_block_n16_nostim", 16, W.FLAGS_ALL,
+         np.array([1, 1, 1, 1, 1, 1, 1, 0], np.int32), 0),
+        ("hr98_med_n20_nostim", 20, W.FLAGS_ALL, np.array([0, 1, 0, 1, 0, 0, 1, 0], np.int32), 0),
+        ("hr98_none_n6_stim", 6, W.FLAGS_ALL, W.params_default_none(W.BREM_HR98), 1),
+        ("gp19_n4_nostim", 4, W.FLAGS_ALL, W.params_default_all(W.BREM_GP19), 0),
+    ]
+    for i in range(5):
+        f = np.zeros(5, np.int32)
+        f[i] = 1
+        cases.append((f"only{i}_n10_nostim", 10, f, np.array([1, 1, 1, 1, 1, 1, 1, 0], np.int32),
+                      0))
+    for tag, nq, fl, prm, stim in cases:
+        variants[f"out_{tag}"] = O.spectral_opacities(mix, nu, nq, fl, prm, stim=stim,
+                                                      kind="ref")
+        variants[f"nquad_{tag}"] = np.int32(nq)
+        variants[f"flags_{tag}"] = fl
+        variants[f"params_{tag}"] = prm
+        variants[f"stim_{tag}"] = np.int32(stim)
+    # every point its own energies
+    rng = np.random.Generator(np.random.PCG64(23))
+    nu_pp = np.exp(rng.uniform(np.log(0.1), np.log(1000.0), size=(5, mix.shape[1])))
+    variants["out_perpoint_n10_stim"] = O.spectral_opacities(
+        mix, nu_pp, 10, W.FLAGS_ALL, W.params_default_all(W.BREM_BRT06), stim=1, kind="ref")
+    # IsoScattLegCoeff (l = 0) on its own, with and without weak magnetism
+    eos9 = np.ascontiguousarray(mix[:9, 20])
+    e_iso = np.exp(np.linspace(np.log(0.05), np.log(400.0), 40))
+    variants["iso_leg0_wm"] = O.iso_leg0(e_iso, eos9, W.params_default_all(1), kind="ref")
+    variants["iso_leg0_nowm"] = O.iso_leg0(e_iso, eos9, W.params_default_none(1), kind="ref")
+    save("spectral_variants.npz", pts=mix, nu=nu, nu_perpoint=nu_pp, iso_eos9=eos9,
+         iso_omega=e_iso, **variants)
+
+
+def main(groups=None):
     if not O.available("ref"):
         raise SystemExit("oracle/_ref/libbnsref.so missing: run make -C oracle where "
                          "/root/reference exists")
+    if groups:
+        for g in groups:
+            {"spectral": spectral_fixtures}[g]()
+        return
 
     # --- Gauss-Legendre (integration.hpp:28-87) ---------------------------
     gl = {}
@@ -180,6 +265,8 @@ def main():
                                                       kind="ref")
     save("m1_variants.npz", pts=mix, pts_eq=meq, distr_eq=deq, eq_params=prm, **variants)
 
+    spectral_fixtures()
+
 
 def _bessk1_ref(x):
     lib, _ = O.load("ref")
@@ -189,4 +276,4 @@ def _bessk1_ref(x):
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
