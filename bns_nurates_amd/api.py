@@ -5,6 +5,9 @@ Mirrors the reference's host interface for the gray M1 path:
 * ``gauss_legendre(n)``             -- ``GaussLegendre`` (integration.hpp:28-87)
 * ``Context.compute_m1_opacities``  -- batched ``ComputeM1Opacities`` /
   ``ComputeM1OpacitiesNotStimulated`` (m1_opacities.hpp:1297-1313)
+* ``Context.compute_spectral_opacities`` -- batched
+  ``ComputeSpectralOpacitiesNotStimulatedAbs`` / ``...StimulatedAbs``
+  (m1_opacities.hpp:1505-1660) over points x energies
 * ``Context.distr_from_m1``         -- ``CalculateDistrParamsFromM1``
   (distribution.hpp:323-333)
 * ``Context.equilibrium``           -- ``NuEquilibriumParams`` +
@@ -28,6 +31,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbnsamd.so")
 
 PT_NFIELDS, DI_NFIELDS, OUT_NFIELDS = 21, 28, 20
+SP_NFIELDS = 16  # SpectralOpacities: j, kappa, j_s, kappa_s per species
 FP64, MIXED = 0, 1
 BREM_HR98, BREM_BRT06, BREM_GP19 = 0, 1, 2
 
@@ -157,6 +161,11 @@ def load_library():
                                  C.c_int, C.c_int, vp], C.c_int),
         "bnsamd_m1_opacities_host": ([vp, i64, vp, vp, C.POINTER(_Flags), C.POINTER(_Params),
                                       C.c_int, C.c_int, vp], C.c_int),
+        "bnsamd_spectral_opacities": ([vp, i64, i64, vp, vp, i64, vp, i64, C.POINTER(_Flags),
+                                       C.POINTER(_Params), C.c_int, vp], C.c_int),
+        "bnsamd_spectral_opacities_host": ([vp, i64, vp, vp, i64, vp, C.c_int,
+                                            C.POINTER(_Flags), C.POINTER(_Params), C.c_int,
+                                            vp], C.c_int),
         "bnsamd_distr_from_m1": ([vp, i64, i64, vp, vp], C.c_int),
         "bnsamd_equilibrium": ([vp, i64, i64, vp, vp], C.c_int),
         "bnsamd_fdi": ([vp, C.c_int, i64, vp, vp], C.c_int),
@@ -324,6 +333,68 @@ class Context:
                                                  int(stim_abs), int(precision), op),
                     "bnsamd_m1_opacities")
         return out
+
+    def compute_spectral_opacities(self, pts, nu, flags=None, params=None, stim_abs: int = 0,
+                                   distr=None, out=None):
+        """Spectral opacities [16, nnu, npts] (j, kappa, j_s, kappa_s of the
+        four species; field order of SpectralOpacities) of the point block
+        ``pts`` [21, npts] at the energies ``nu``: [nnu] shared by all points
+        or [nnu, npts] per point.  The partner-energy integrals use the 1D
+        quadrature of ``set_quadrature``.  stim_abs = 0 is
+        ComputeSpectralOpacitiesNotStimulatedAbs, 1 ...StimulatedAbs.
+
+        Device tensors run asynchronously on the context stream (``out`` a
+        [16 * nnu, ld] tensor, allocated when None, returned viewed as
+        [16, nnu, ld]); numpy arrays go through the synchronous host entry
+        point."""
+        fl, pr = as_flags(flags).c(), as_params(params).c()
+        if isinstance(pts, np.ndarray):
+            pts = np.ascontiguousarray(pts, dtype=np.float64)
+            n = pts.shape[1]
+            nu = np.ascontiguousarray(nu, dtype=np.float64)
+            if nu.ndim == 1:
+                per_point = 0
+            elif nu.ndim == 2 and nu.shape[1] == n:
+                per_point = 1
+            else:
+                raise BnsAmdError("nu must be [nnu] or [nnu, npts]")
+            nnu = nu.shape[0]
+            res = np.empty((SP_NFIELDS, nnu, n))
+            dd = None
+            if distr is not None:
+                distr = np.ascontiguousarray(distr, dtype=np.float64)
+                dd = distr.ctypes.data
+            self._check(self.lib.bnsamd_spectral_opacities_host(
+                self.h, n, pts.ctypes.data, dd, nnu, nu.ctypes.data, per_point, C.byref(fl),
+                C.byref(pr), int(stim_abs), res.ctypes.data), "bnsamd_spectral_opacities_host")
+            return res
+        import torch
+        self._follow_torch_stream()
+        n = pts.shape[1]
+        pp, ld = _tensor_ptr(pts, PT_NFIELDS, n, "pts")
+        if nu.dtype != torch.float64 or not nu.is_cuda or not nu.is_contiguous():
+            raise BnsAmdError("nu: contiguous float64 device tensor required")
+        if nu.dim() == 1:
+            nu_ld = 0
+        elif nu.dim() == 2 and nu.shape[1] >= n:
+            nu_ld = nu.stride(0)
+        else:
+            raise BnsAmdError("nu must be [nnu] or [nnu, >= npts]")
+        nnu = nu.shape[0]
+        dp = None
+        if distr is not None:
+            dp, ldd = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+            if ldd != ld:
+                raise BnsAmdError("distr must have the leading dimension of pts")
+        if out is None:
+            out = torch.empty((SP_NFIELDS * nnu, ld), dtype=torch.float64, device=pts.device)
+        op, ldo = _tensor_ptr(out, SP_NFIELDS * nnu, n, "out")
+        if ldo != ld:
+            raise BnsAmdError("out must have the leading dimension of pts")
+        self._check(self.lib.bnsamd_spectral_opacities(
+            self.h, n, ld, pp, dp, nnu, nu.data_ptr(), nu_ld, C.byref(fl), C.byref(pr),
+            int(stim_abs), op), "bnsamd_spectral_opacities")
+        return out.view(SP_NFIELDS, nnu, ld)
 
     def distr_from_m1(self, pts, distr=None):
         import torch

@@ -15,6 +15,7 @@
 #include "gray_kernel.h"
 #include "gray_math.h"
 #include "gray_tables.h"
+#include "spectral_kernel.h"
 
 namespace {
 
@@ -221,6 +222,49 @@ int run_gray(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pts,
   return BNSAMD_OK;
 }
 
+int run_spectral(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pts,
+                 const double* d_distr, int64_t nnu, const double* d_nu, int64_t nu_ld,
+                 const bnsamd_flags* flags, const bnsamd_params* params, int stim_abs,
+                 double* d_out) {
+  if (npts < 0 || ld < npts || nnu < 0 || !d_pts || !d_nu || !d_out || !flags || !params ||
+      (nu_ld != 0 && nu_ld < npts))
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities: invalid argument");
+  if (stim_abs != 0 && stim_abs != 1)
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities: stim_abs must be 0 or 1");
+  if (params->brem_implementation < 0 || params->brem_implementation > 2)
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities: unknown brem_implementation");
+  if (!c->current) return fail(c, BNSAMD_ERR_NO_QUADRATURE, "bnsamd_set_quadrature not called");
+  if (npts == 0 || nnu == 0) return BNSAMD_OK;
+  const TableLayout& L = c->current->L;
+  bns::SpectralKernelArgs a{};
+  a.npts = npts;
+  a.ld = ld;
+  a.pts = d_pts;
+  a.distr = d_distr;
+  a.nnu = nnu;
+  a.nu = d_nu;
+  a.nu_ld = nu_ld;
+  a.out = d_out;
+  a.q1 = c->current->dev + L.q1;
+  a.n = L.n1;
+  a.ech = bns::spectral_energies_per_group(L.n1);
+  a.nchunk = (nnu + a.ech - 1) / a.ech;
+  if (npts > int64_t(0x7fffffff) / a.nchunk)
+    return fail(c, BNSAMD_ERR_INVALID,
+                "bnsamd_spectral_opacities: npts x ceil(nnu / energies per group) above 2^31 - 1");
+  a.flags = bns::PointFlags{flags->use_abs_em, flags->use_pair, flags->use_brem,
+                            flags->use_inelastic_scatt, flags->use_iso};
+  a.params = bns::PointParams{params->use_dU, params->use_dm_eff, params->use_WM_ab,
+                              params->use_WM_sc, params->use_decay,
+                              params->brem_implementation, params->use_NN_medium_corr,
+                              params->neglect_blocking};
+  a.stim = stim_abs;
+  a.gamma36 = bns::gamma_stirling_36();
+  hipError_t e = bns::launch_spectral_f64(a, c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "spectral kernel launch");
+  return BNSAMD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -387,6 +431,54 @@ int bnsamd_m1_opacities_host(bnsamd_context* c, int64_t npts, const double* h_pt
                      hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "bnsamd_m1_opacities_host: download");
+  return BNSAMD_OK;
+}
+
+int bnsamd_spectral_opacities(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pts,
+                              const double* d_distr, int64_t nnu, const double* d_nu,
+                              int64_t nu_ld, const bnsamd_flags* flags,
+                              const bnsamd_params* params, int stim_abs, double* d_out) {
+  if (check_ctx(c)) return BNSAMD_ERR_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  return run_spectral(c, npts, ld, d_pts, d_distr, nnu, d_nu, nu_ld, flags, params, stim_abs,
+                      d_out);
+}
+
+int bnsamd_spectral_opacities_host(bnsamd_context* c, int64_t npts, const double* h_pts,
+                                   const double* h_distr, int64_t nnu, const double* h_nu,
+                                   int per_point_energies, const bnsamd_flags* flags,
+                                   const bnsamd_params* params, int stim_abs, double* h_out) {
+  if (check_ctx(c)) return BNSAMD_ERR_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  if (npts < 0 || nnu < 0 || !h_pts || !h_nu || !h_out ||
+      (per_point_energies != 0 && per_point_energies != 1))
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities_host: invalid argument");
+  if (npts == 0 || nnu == 0) return BNSAMD_OK;
+  (void)hipSetDevice(c->device);
+  const size_t nnu_vals = size_t(nnu) * (per_point_energies ? size_t(npts) : 1);
+  const size_t nout = size_t(npts) * size_t(nnu) * BNSAMD_SP_NFIELDS;
+  size_t need = size_t(npts) * (BNSAMD_PT_NFIELDS + (h_distr ? BNSAMD_DI_NFIELDS : 0)) +
+                nnu_vals + nout;
+  if (int rc = ensure_stage(c, need, "bnsamd_spectral_opacities_host")) return rc;
+  double* dp = c->stage;
+  double* dout = dp + size_t(npts) * BNSAMD_PT_NFIELDS;
+  double* dnu = dout + nout;
+  double* dd = h_distr ? dnu + nnu_vals : nullptr;
+  hipError_t e = hipMemcpyAsync(dp, h_pts, sizeof(double) * npts * BNSAMD_PT_NFIELDS,
+                                hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dnu, h_nu, sizeof(double) * nnu_vals, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && dd)
+    e = hipMemcpyAsync(dd, h_distr, sizeof(double) * npts * BNSAMD_DI_NFIELDS,
+                       hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "bnsamd_spectral_opacities_host: upload");
+  int rc = run_spectral(c, npts, npts, dp, dd, nnu, dnu, per_point_energies ? npts : 0, flags,
+                        params, stim_abs, dout);
+  if (rc != BNSAMD_OK) return rc;
+  e = hipMemcpyAsync(h_out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "bnsamd_spectral_opacities_host: download");
   return BNSAMD_OK;
 }
 

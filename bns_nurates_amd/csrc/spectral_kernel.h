@@ -1,0 +1,43 @@
+// spectral_kernel.h -- launch arguments of the spectral opacity kernel.
+#ifndef BNS_NURATES_AMD_SPECTRAL_KERNEL_H_
+#define BNS_NURATES_AMD_SPECTRAL_KERNEL_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "gray_kernel.h"
+
+namespace bns {
+
+struct SpectralKernelArgs {
+  int64_t npts;          // points in this launch
+  int64_t ld;            // elements between consecutive fields of the point blocks
+  const double* pts;     // [21][ld] point block (device)
+  const double* distr;   // [28][ld] distribution block, or nullptr: reconstructed from M1
+  int64_t nnu;           // energies per point
+  const double* nu;      // energies [MeV]: nu[e] (nu_ld == 0) or nu[e * nu_ld + i]
+  int64_t nu_ld;
+  double* out;           // [16][nnu][ld]: out[(f * nnu + e) * ld + i]
+  const double* q1;      // 1D quadrature records (x, w) on [0, 1] (device)
+  int n;                 // 1D quadrature order, 1..32
+  int ech;               // energies per workgroup
+  int64_t nchunk;        // workgroups per point, ceil(nnu / ech)
+  PointFlags flags;
+  PointParams params;
+  int stim;              // 1: ComputeSpectralOpacitiesStimulatedAbs, 0: ...NotStimulatedAbs
+  double gamma36;        // GammaStirling(3.6)
+};
+
+// Energies per workgroup for quadrature order n (a multiple of the energies
+// one round of cells covers, floor(64 / 2n)).
+int spectral_energies_per_group(int n);
+// Dynamic LDS bytes of one workgroup.
+size_t spectral_lds_bytes(int n, int ech);
+
+hipError_t launch_spectral_f64(const SpectralKernelArgs& a, hipStream_t stream);
+
+}  // namespace bns
+
+#endif  // BNS_NURATES_AMD_SPECTRAL_KERNEL_H_

@@ -1,11 +1,11 @@
 /* bns_nurates_amd.h -- C ABI of the MI355X gray M1 neutrino-opacity library.
  *
  * Drop-in boundary for the gray (energy-integrated) M1 opacity path of
- * bns_nurates.  Every entry point is plain C: pointers, sizes and ints; no
- * HIP or torch types.  Device pointers are HIP device addresses; streams are
- * passed as `void*` (a hipStream_t).  Errors are returned as status codes,
- * never by abort(); bnsamd_last_error() gives the message of the context's
- * last failure.
+ * bns_nurates, and for its spectral (energy-dependent) opacities.  Every
+ * entry point is plain C: pointers, sizes and ints; no HIP or torch types.
+ * Device pointers are HIP device addresses; streams are passed as `void*` (a
+ * hipStream_t).  Errors are returned as status codes, never by abort();
+ * bnsamd_last_error() gives the message of the context's last failure.
  *
  * Data layout (SoA, one double per point per field, field f of point i at
  * base[f * ld + i], ld >= npts):
@@ -56,6 +56,14 @@ enum {
 enum {
   BNSAMD_OUT_ETA_0 = 0, BNSAMD_OUT_KAPPA_0_A = 4, BNSAMD_OUT_ETA = 8,
   BNSAMD_OUT_KAPPA_A = 12, BNSAMD_OUT_KAPPA_S = 16, BNSAMD_OUT_NFIELDS = 20
+};
+
+/* spectral output block: SpectralOpacities (bns_nurates.hpp:517-523), one
+ * field per (quantity, species), each field [nnu][ld]: value of energy e at
+ * point i at base[(f * nnu + e) * ld + i] */
+enum {
+  BNSAMD_SP_J = 0, BNSAMD_SP_KAPPA = 4, BNSAMD_SP_J_S = 8, BNSAMD_SP_KAPPA_S = 12,
+  BNSAMD_SP_NFIELDS = 16
 };
 
 /* OpacityFlags (bns_nurates.hpp:418-426), same field order and types */
@@ -121,6 +129,30 @@ int bnsamd_m1_opacities_host(bnsamd_context* ctx, int64_t npts, const double* h_
                              const double* h_distr, const bnsamd_flags* flags,
                              const bnsamd_params* params, int stim_abs, int precision,
                              double* h_out);
+
+/* Spectral opacities of npts points at nnu neutrino energies each, device
+ * buffers, asynchronous on the context stream.  Replaces
+ * ComputeSpectralOpacitiesNotStimulatedAbs (stim_abs = 0) and
+ * ComputeSpectralOpacitiesStimulatedAbs (stim_abs = 1),
+ * m1_opacities.hpp:1505-1660, called once per (energy, point) by the
+ * reference's drivers (tests/tests.hpp:1174-1176, mwe.cpp:232,312).  The
+ * integrals over the partner energy use the context's 1D quadrature
+ * (n1 of bnsamd_set_quadrature, the reference's quad_1d).  Energies [MeV]:
+ * d_nu[e] for every point when nu_ld == 0, else d_nu[e * nu_ld + i]
+ * (nu_ld >= npts).  d_out: BNSAMD_SP_* fields, 16 * nnu rows of ld doubles.
+ * d_distr as for bnsamd_m1_opacities (NULL: reconstructed from n, J, chi). */
+int bnsamd_spectral_opacities(bnsamd_context* ctx, int64_t npts, int64_t ld, const double* d_pts,
+                              const double* d_distr, int64_t nnu, const double* d_nu,
+                              int64_t nu_ld, const bnsamd_flags* flags,
+                              const bnsamd_params* params, int stim_abs, double* d_out);
+
+/* Same, from and to host memory (synchronous; ld = npts).  h_nu holds nnu
+ * energies shared by all points (per_point_energies = 0) or nnu * npts
+ * (per_point_energies = 1, energy e of point i at h_nu[e * npts + i]). */
+int bnsamd_spectral_opacities_host(bnsamd_context* ctx, int64_t npts, const double* h_pts,
+                                   const double* h_distr, int64_t nnu, const double* h_nu,
+                                   int per_point_energies, const bnsamd_flags* flags,
+                                   const bnsamd_params* params, int stim_abs, double* h_out);
 
 /* CalculateDistrParamsFromM1 (distribution.hpp:323-333) on the device:
  * d_distr[28][ld] from the n, J, chi fields of d_pts. */
