@@ -9,8 +9,11 @@ Prints one JSON line (the driver's contract):
 
 Beside the points/s it reports the roofline of the fused kernel (algorithmic
 FP64 FLOPs per launch / kernel time against the FP64 vector peak), the HR98
-variant, and the CPU baseline: the CPU oracle timed on the host cores on a
-bounded sample of the same workload.
+variant, the mixed-precision form, the spectral path
+(ComputeSpectralOpacitiesNotStimulatedAbs over 32^3 points x 100 energies,
+with its own roofline and CPU baseline), the serving sweep, the learned form,
+and the CPU baseline: the CPU oracle timed on the host cores on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -47,6 +50,19 @@ ALGORITHMIC_BYTES_PER_POINT = 8 * (21 + 20)
 # hardware FP64 FLOPs per point, (ADD + MUL + TRANS + 2 FMA) x 64 lanes from
 # the SQ_INSTS_VALU_*_F64 counters over the 2,097,152-point launch
 HW_FLOPS_PER_POINT = 1943232.0
+
+
+# The spectral path (ComputeSpectralOpacitiesNotStimulatedAbs): 32^3 points
+# of the grid workload x the reference spectral test's 100 energies
+# 10^(-1 + 4k/100) MeV (tests/tests.hpp:882-894), Gauss-Legendre n = 10, all
+# reactions, BRT06, default_all corrections.  Algorithmic work per (point,
+# energy) evaluation from the counting build of the reuse-aware restatement
+# on this workload (tests/test_bench_constants.py re-derives it).
+SPECTRAL_NPTS = 32 ** 3
+SPECTRAL_NQUAD = 10
+SPECTRAL_NU = 10.0 ** (-1.0 + 4.0 * np.arange(100) / 100.0)
+SPECTRAL_FLOPS_PER_EVAL = 25387.1
+SPECTRAL_TRANSCENDENTALS_PER_EVAL = {"exp": 440.5, "pow": 109.9, "log": 7.6}
 
 
 # outputs whose accuracy the reduced-precision form reports: number and
@@ -246,6 +262,85 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0):
+    """The spectral leg: events-timed passes of the spectral kernel over the
+    spectral workload (inputs resident in HBM), an oracle parity sample, and
+    the CPU restatement timed on a bounded prefix of the points."""
+    import torch
+    import oracle as O
+    from bns_nurates_amd import workloads as W
+    sp = W.make_points(SPECTRAL_NPTS, seed=1)
+    npts, nnu = sp.shape[1], len(SPECTRAL_NU)
+    dsp = torch.from_numpy(sp).cuda()
+    dnu = torch.from_numpy(SPECTRAL_NU).cuda()
+    prm = W.params_default_all(W.BREM_BRT06)
+    ctx.set_quadrature(SPECTRAL_NQUAD)
+    out = torch.empty((16 * nnu, npts), dtype=torch.float64, device="cuda")
+
+    def run():
+        ctx.compute_spectral_opacities(dsp, dnu, W.FLAGS_ALL, prm, stim_abs=0, out=out)
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    log(f"spectral: timing {steps} steps ({npts} points x {nnu} energies)")
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        run()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    nev = npts * nnu
+    achieved = SPECTRAL_FLOPS_PER_EVAL * nev / (kern_ms * 1e-3) / 1e12
+    o = out.view(16, nnu, npts).cpu().numpy()
+    assert np.isfinite(o).all()
+    idx = np.arange(0, npts, npts // 32)[:32]
+    ref, mag = O.bound_spectral_opacities(np.ascontiguousarray(sp[:, idx]), SPECTRAL_NU,
+                                          SPECTRAL_NQUAD, W.FLAGS_ALL, prm, stim=0)
+    excess = float(O.parity_excess(o[:, :, idx], ref, mag).max())
+    res = {"workload": f"spectral_grid32_e100_n10_brt06: {npts} points of the grid workload x "
+                       f"{nnu} energies 10^(-1+4k/100) MeV (the reference spectral test's grid), "
+                       "n=10, all reactions, BRT06, default_all, not stimulated, distributions "
+                       "reconstructed from M1",
+           "value": nev * steps / wall, "unit": "(point, energy) evaluations/s",
+           "steps": steps, "kernel_ms": kern_ms,
+           "roofline": {"bound": "fp64 vector ALU", "achieved": achieved,
+                        "peak": PEAK_FP64_VECTOR_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_VECTOR_TFLOPS, "kernel": "k_spectral_f64",
+                        "flops_per_eval": SPECTRAL_FLOPS_PER_EVAL,
+                        "transcendentals_per_eval": SPECTRAL_TRANSCENDENTALS_PER_EVAL,
+                        "algorithmic_bytes_per_eval": 16 * 8 + 8 * 21 / nnu},
+           "parity_sample": {"points": len(idx), "energies": nnu, "bar_excess_max": excess}}
+    log(f"spectral: {res['value']:.4g} evals/s, kernel {kern_ms:.2f} ms, "
+        f"{achieved:.2f} TFLOP/s, parity excess {excess:.3g}")
+    if cpu:
+        kind = "fast" if O.available("fast") else "orc"
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        probe = np.ascontiguousarray(sp[:, :2 * threads])
+        t0 = time.perf_counter()
+        O.spectral_opacities(probe, SPECTRAL_NU, SPECTRAL_NQUAD, W.FLAGS_ALL, prm, stim=0,
+                             kind=kind, nthreads=threads)
+        rate = probe.shape[1] / (time.perf_counter() - t0)
+        n = int(min(max(rate * target_s, threads), npts))
+        sample = np.ascontiguousarray(sp[:, :n])
+        t0 = time.perf_counter()
+        O.spectral_opacities(sample, SPECTRAL_NU, SPECTRAL_NQUAD, W.FLAGS_ALL, prm, stim=0,
+                             kind=kind, nthreads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n * nnu / dt, "unit": "(point, energy) evaluations/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"first {n} points x {nnu} energies, {dt:.1f} s wall on "
+                                         f"{threads} threads, reuse-aware restatement "
+                                         "oracle/m1_oracle.hpp built -O3 -march=x86-64-v3"}
+        res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        log(f"spectral CPU baseline: {res['cpu_baseline']['value']:.4g} evals/s")
+    return res
+
+
 def cpu_baseline(pts, nquad, params, target_s=10.0):
     """The CPU restatement of the path (FP64, OpenMP over points, built with
     -O3 for AVX2/FMA) on a bounded prefix of the workload, all threads the
@@ -284,6 +379,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision passes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned form")
+    ap.add_argument("--no-spectral", action="store_true", help="skip the spectral leg")
     ap.add_argument("--surrogate-train-points", type=int, default=10_000_000)
     ap.add_argument("--surrogate-epochs", type=int, default=8)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -419,6 +515,9 @@ def main():
             "speedup_vs_fp64": res[brem]["kernel_ms"] / h["kernel_ms"] if brem in res else None,
             "fp32_parts": "distribution values (exp/log), HR98 channel functions",
             "rel_err_vs_fp64_oracle": acc}
+    if not args.no_spectral:
+        line["spectral"] = spectral_eval(ctx, max(3, args.steps // 2), 1,
+                                         cpu=not args.no_cpu)
     if not args.no_serving:
         line["serving"] = serving_sweep()
     if not args.no_surrogate:

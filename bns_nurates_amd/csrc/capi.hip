@@ -247,11 +247,11 @@ int run_spectral(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pt
   a.out = d_out;
   a.q1 = c->current->dev + L.q1;
   a.n = L.n1;
-  a.ech = bns::spectral_energies_per_group(L.n1);
+  a.ech = bns::spectral_energies_per_chunk(L.n1);
   a.nchunk = (nnu + a.ech - 1) / a.ech;
-  if (npts > int64_t(0x7fffffff) / a.nchunk)
-    return fail(c, BNSAMD_ERR_INVALID,
-                "bnsamd_spectral_opacities: npts x ceil(nnu / energies per group) above 2^31 - 1");
+  a.cpw = bns::spectral_chunks_per_group(npts, a.nchunk);
+  if (npts > int64_t(0x7fffffff) / ((a.nchunk + a.cpw - 1) / a.cpw))
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities: too many workgroups");
   a.flags = bns::PointFlags{flags->use_abs_em, flags->use_pair, flags->use_brem,
                             flags->use_inelastic_scatt, flags->use_iso};
   a.params = bns::PointParams{params->use_dU, params->use_dm_eff, params->use_WM_ab,

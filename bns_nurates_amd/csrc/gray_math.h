@@ -111,6 +111,15 @@ BNS_HD double neps_exp_func(double x) {
   return ((neg ? 1. : 0.) - (neg ? 0. : 1.) * ex) / (1. - ex + (x == 0. ? 1. : 0.));
 }
 
+// NEPSExpFunc(x) and NEPSExpFunc(x2) for x2 = -x formed as a separate
+// difference (w - w' and w' - w): one exponential, the same bits as two calls
+BNS_HD void neps_exp_func_pair(double x, double x2, double& f, double& f2) {
+  double ex = safe_exp(-fabs(x));
+  double den = 1. - ex + (x == 0. ? 1. : 0.);
+  f = (std::signbit(x) ? 1. : -ex) / den;
+  f2 = (std::signbit(x2) ? 1. : -ex) / den;
+}
+
 // bessi1 / bessk1: the Abramowitz & Stegun polynomial fits of
 // functions.hpp:99-180 (used by BRT06 bremsstrahlung)
 BNS_HD double bessi1_fit(double x) {

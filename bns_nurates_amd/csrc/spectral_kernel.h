@@ -22,17 +22,22 @@ struct SpectralKernelArgs {
   double* out;           // [16][nnu][ld]: out[(f * nnu + e) * ld + i]
   const double* q1;      // 1D quadrature records (x, w) on [0, 1] (device)
   int n;                 // 1D quadrature order, 1..32
-  int ech;               // energies per workgroup
-  int64_t nchunk;        // workgroups per point, ceil(nnu / ech)
+  int ech;               // energies per chunk (the LDS energy block)
+  int64_t nchunk;        // chunks per point, ceil(nnu / ech)
+  int64_t cpw;           // chunks per workgroup; ceil(nchunk / cpw) workgroups per point
   PointFlags flags;
   PointParams params;
   int stim;              // 1: ComputeSpectralOpacitiesStimulatedAbs, 0: ...NotStimulatedAbs
   double gamma36;        // GammaStirling(3.6)
 };
 
-// Energies per workgroup for quadrature order n (a multiple of the energies
-// one round of cells covers, floor(64 / 2n)).
-int spectral_energies_per_group(int n);
+// Energies per chunk for quadrature order n (a multiple of the energies one
+// round of cells covers, floor(64 / 2n)).
+int spectral_energies_per_chunk(int n);
+// Chunks per workgroup: all chunks of a point in one workgroup (its point
+// tables are built once) unless that leaves too few workgroups to fill the
+// device.
+int64_t spectral_chunks_per_group(int64_t npts, int64_t nchunk);
 // Dynamic LDS bytes of one workgroup.
 size_t spectral_lds_bytes(int n, int ech);
 

@@ -122,16 +122,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 
   const int lane = threadIdx.x;
   const int n = A.n, N = 2 * n, E = A.ech;
-  // workgroup -> (point, energy chunk); consecutive points of one chunk share
-  // an XCD (and its L2), as in the gray kernel
+  // workgroup -> (point, group of cpw energy chunks); consecutive points of
+  // one group share an XCD (and its L2), as in the gray kernel
   const int64_t b = xcd_point(blockIdx.x, gridDim.x);
-  const int64_t p = b % A.npts, chunk = b / A.npts;
-  const int64_t e0 = chunk * E;
-  const int ne = static_cast<int>(A.nnu - e0 < E ? A.nnu - e0 : E);
+  const int64_t p = b % A.npts, grp = b / A.npts;
+  const int64_t c_begin = grp * A.cpw;
+  const int64_t c_end = c_begin + A.cpw < A.nchunk ? c_begin + A.cpw : A.nchunk;
   const int64_t ld = A.ld;
   const SpLds L = sp_lds(n, E);
   const volatile double* vin = s_in;
-  volatile double* V = sm;
+  // the LDS block: written and read in phases separated by barriers
+  double* V = sm;
 
   // ---- inputs -----------------------------------------------------------------
   if (lane < 21) s_in[lane] = A.pts[lane * ld + p];
@@ -201,46 +202,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   // partner node a of the pair/brem integral: t x_a or t / x_{a-n}
   // (integration.hpp:525-534), formed with the reference's rounding
   auto pair_node = [&](int a) { return a < n ? mul_rn(tpair, s_x[a]) : tpair / s_x[a - n]; };
-  auto energy = [&](int e) {
-    return A.nu_ld ? A.nu[(e0 + e) * A.nu_ld + p] : A.nu[e0 + e];
-  };
-
-  // ---- Fermi-integral bundles: F_1..F_5 at eta -+ z_a (pair nodes) and at
-  //      eta - y, eta + y, -eta - y (energies) -----------------------------------
+  // ---- pair/brem node tables (once per point): F_1..F_5 at eta -+ z_a,
+  //      TotalNuF at the node and the blocking factor --------------------------
   {
-    const int nnode = do_pair ? 2 * N : 0;
-    const int ntask = nnode + 3 * ne;
+    const int nbund = do_pair ? 2 * N : 0;
+    const int ntask = do_pb ? nbund + N : 0;
     for (int t = lane; t < ntask; t += kWave) {
-      double arg;
-      int dst, stride;
-      if (t < nnode) {
+      if (t < nbund) {
         const int a = t < N ? t : t - N;
         const double z = pair_node(a) / T;
-        arg = t < N ? eta - z : eta + z;
-        dst = (t < N ? L.fzm : L.fzp) + a;
-        stride = N;
-      } else {
-        const int u = t - nnode, e = u / 3, kind = u - 3 * e;
-        const double y = energy(e) / T;
-        arg = kind == 0 ? eta - y : kind == 1 ? eta + y : -eta - y;
-        dst = (kind == 0 ? L.fym : kind == 1 ? L.fyp : L.fyq) + e;
-        stride = E;
-      }
-      double F[6];
-      fdi_bundle<1, 5>(arg, F);
+        double F[6];
+        fdi_bundle<1, 5>(t < N ? eta - z : eta + z, F);
+        const int dst = (t < N ? L.fzm : L.fzp) + a;
 #pragma unroll
-      for (int k = 1; k <= 5; ++k) V[dst + (k - 1) * stride] = F[k];
-    }
-  }
-  // ---- distribution values, beta and iso -------------------------------------------
-  {
-    const int nnode = do_pb ? N : 0;
-    const int ntask = nnode + 3 * ne;
-    const PointParams& pp = A.params;
-    for (int t = lane; t < ntask; t += kWave) {
-      if (t < nnode) {
-        // TotalNuF at the pair/brem node and the blocking factor
-        const int a = t;
+        for (int k = 1; k <= 5; ++k) V[dst + (k - 1) * N] = F[k];
+      } else {
+        const int a = t - nbund;
         const double nub = pair_node(a);
         V[L.nub + a] = nub;
         V[L.z + a] = nub / T;
@@ -251,300 +228,360 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
           V[L.gz + s * N + a] = g;
           V[L.bz + s * N + a] = blocking ? 1. - g : 1.;
         }
-      } else {
-        const int u = t - nnode, e = u / 3, kind = u - 3 * e;
-        const double nu = energy(e);
-        if (kind == 0) {
-          // TotalNuF(nu) of the four species (m1_opacities.hpp:1533-1536)
-          V[L.nu + e] = nu;
-          V[L.y + e] = nu / T;
-          const double lw = log(nu);
-#pragma unroll
-          for (int s = 0; s < 4; ++s) V[L.gnu + s * E + e] = nuf(nu, lw, Dget(s));
-        } else if (kind == 1) {
-          // AbsOpacity (formulas_beta_iso.hpp:291-319), nue and anue
-          double em[2] = {0., 0.}, ab[2] = {0., 0.};
-          if (A.flags.use_abs_em != 0) {
-            const BetaPoint bp =
-                beta_point(vin[0], T, vin[2], vin[3], vin[4], vin[5], mu_e, vin[7], vin[8],
-                           pp.use_dU != 0, pp.use_dm_eff != 0, pp.use_WM_ab != 0,
-                           pp.use_decay != 0);
-            beta_em_ab(nu, 0, bp, em[0], ab[0]);
-            beta_em_ab(nu, 1, bp, em[1], ab[1]);
-          }
-#pragma unroll
-          for (int s = 0; s < 2; ++s) {
-            V[L.bem + s * E + e] = em[s];
-            V[L.bab + s * E + e] = ab[s];
-          }
-        } else {
-          // IsoScattLegCoeff, l = 0 (formulas_beta_iso.hpp:524-560)
-          double iso = 0.;
-          if (A.flags.use_iso != 0) {
-            const double etap = eta_nn_sc(vin[0], T, vin[2]), etan = eta_nn_sc(vin[0], T, vin[3]);
-            double r0p = 1., r0n = 1.;
-            if (pp.use_WM_sc) {
-              r0p = wm_scatt_r0(nu, 1);
-              r0n = wm_scatt_r0(nu, 2);
-            }
-            iso = etap * c::IsoC0p * r0p + etan * c::IsoC0n * r0n;
-          }
-          V[L.iso + e] = iso;
-        }
       }
     }
   }
-  __syncthreads();
 
-  // per-energy quadrature sums of one round: lane (g, k) sums integrand k of
-  // energy r0 + g over the first and second half of the nodes in node order
-  // (DoIntegration, integration.hpp:95-108) and scales by t
-  const int Gr = kWave / N;  // energies per round
-  auto reduce_round = [&](int r0, int dst, bool neps) {
-    for (int j = lane; j < 8 * Gr; j += kWave) {
-      const int g = j >> 3, k = j & 7, e = r0 + g;
-      if (e >= ne) continue;
-      const int base = L.sf + k * kWave + g * N;
-      double s1 = 0., s2 = 0.;
-      for (int i = 0; i < n; ++i) s1 = fma(s_w[i], V[base + i], s1);
-      for (int i = 0; i < n; ++i) s2 = fma(s_w[i], V[base + n + i], s2);
-      const double t = neps ? V[L.nu + e] : tpair;
-      V[dst + k * E + e] = t * (s1 + s2);
-    }
-  };
-
-  // ====================== pair + bremsstrahlung integral ========================
   const double a1e = c::PairA1E * c::PairA1E, a2e = c::PairA2E * c::PairA2E;
   const double a1x = c::PairA1X * c::PairA1X, a2x = c::PairA2X * c::PairA2X;
-  if (do_pb) {
-    for (int r0 = 0; r0 < ne; r0 += Gr) {
-      const int g = lane / N, a = lane - g * N, e = r0 + g;
-      if (g < Gr && e < ne) {
-        const double nu = V[L.nu + e], y = V[L.y + e];
-        const double nub = V[L.nub + a], z = V[L.z + a];
-        double em[4] = {0., 0., 0., 0.}, ab[4] = {0., 0., 0., 0.};
-        if (do_pair) {
-          // PairKernels with cos_theta = 1, lmax = 0 (kernel_pair.hpp:135-198)
-          double psi0 = 0., psi1 = 0.;
-          const double emyz = (eta - y) - z;
-          if (!(emyz > 200.)) {
-            // PairPsi (kernel_pair.hpp:16-120)
-            double Fm3[6], Fp3[6];
-            fdi_bundle<3, 5>(emyz, Fm3);
-            fdi_bundle<3, 5>((eta + y) + z, Fp3);
-            const int ey = e, az = a;
-            auto fym = [&](int k) { return V[L.fym + (k - 1) * E + ey]; };
-            auto fyp = [&](int k) { return V[L.fyp + (k - 1) * E + ey]; };
-            auto fzm = [&](int k) { return V[L.fzm + (k - 1) * N + az]; };
-            auto fzp = [&](int k) { return V[L.fzp + (k - 1) * N + az]; };
-            double x0 = 20. * fym(4), x1 = 20. * fzp(4);
-            double x2 = 120. * (fym(2) - fzp(2));
-            double x3 = 40. * (fzp(3) - fym(3));
-            double x4 = 40. * vFe[0][3];
-            double x5 = 40. * Fm3[3] - x4;
-            double x6 = -20. * vFe[0][4];
-            double x7 = 20. * Fm3[4] + x6;
-            double x8 = x4 - 40. * Fp3[3];
-            double x9 = 20. * Fp3[4] + x6;
-            double x10 = 4. * (((((Fm3[5] - fym(5)) - fzm(5)) + fyp(5)) - Fp3[5]) + fzp(5));
-            double x11 = 20. * fyp(4), x12 = 20. * fzm(4);
-            double x13 = 40. * (fyp(3) - fzm(3));
-            double x14 = 120. * (fzm(2) - fyp(2));
-            double yz = y * z;
-            double aux15 = 15. * (yz * yz);
-            psi0 = x10 +
-                   y * (-x0 + x1 + x7 +
-                        y * (x3 + x5 + z * (x2 + z * (120. * (fzp(1) - fym(1))))) +
-                        z * (80. * (fym(3) - fzp(3)) - x2 * z)) +
-                   z * (x0 - x1 + x9 + z * (x3 + x8));
-            psi1 = x10 + y * (-x11 + x12 + x9 + y * (x13 + x8)) +
-                   z * (x11 - x12 + x7 + y * (80. * (fzm(3) - fyp(3)) - x14 * y) +
-                        z * (x13 + x5 + y * (x14 + y * (120. * (fyp(1) - fzm(1))))));
-            psi0 /= aux15;
-            psi1 /= aux15;
-          }
-          // PairPhi: Phi_0 = (a1^2 psi0 + a2^2 psi1) C T^2 / (1 - e^{y+z}),
-          // em = Phi_0 / 2, abs = e^{(w+w')/T} em
-          const double aux = c::PairPhiC * (T * T) / (1. - safe_exp(y + z));
-          const double db = safe_exp((nu + nub) / T);
-          em[0] = 0.5 * ((a1e * psi0 + a2e * psi1) * aux);
-          em[1] = 0.5 * ((a1e * psi1 + a2e * psi0) * aux);
-          em[2] = 0.5 * ((a1x * psi0 + a2x * psi1) * aux);
-          em[3] = 0.5 * ((a1x * psi1 + a2x * psi0) * aux);
-#pragma unroll
-          for (int s = 0; s < 4; ++s) ab[s] = db * em[s];
-        }
-        double se = 0., sa = 0.;
-        if (do_brem) {
-          if (A.params.brem_impl == 1) {
-            // BremKernelsBRT06 at x = (w + w') / (2T) (kernel_brem_BRT06.hpp:42-69)
-            const double xb = 0.5 * (nu + nub) / T;
-            const double k1x = bessk1_fit(xb) / xb;
-            se = brt_pre * k1x * safe_exp(-xb);
-            sa = brt_pre * k1x * safe_exp(xb);
-          } else {
-            const double x = (nu + nub) / T;
-            sa = hr98_abs(x, hp, s_hc);
-            se = sa * safe_exp(-x);
-          }
-        }
-        // production: partner blocking; annihilation: partner occupation
-        // (SpectralIntegrand, m1_opacities.hpp:1421-1496)
-        const double nb2 = nub * nub;
-        const double jac = a < n ? 1. : 1. / (s_x[a - n] * s_x[a - n]);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int ps = s ^ 1;
-          const double pro = (em[s] + se) * V[L.bz + ps * N + a];
-          const double ann = (ab[s] + sa) * V[L.gz + ps * N + a];
-          V[L.sf + s * kWave + lane] = (nb2 * pro) * jac;
-          V[L.sf + (4 + s) * kWave + lane] = (nb2 * ann) * jac;
-        }
-      }
-      __syncthreads();
-      reduce_round(r0, L.ip, false);
-      __syncthreads();
-    }
-  } else {
-    for (int j = lane; j < 8 * E; j += kWave) V[L.ip + j] = 0.;
-  }
+  const int Gr = kWave / N;  // energies per round of cells
 
-  // ================= inelastic scattering on e-+ (NES + NPS) =====================
-  if (do_neps) {
-    const double pre = c::NEPSConst * (T * T);
-    const double bpl = c::NEPSBPlus, bze = c::NEPSBZero, bmi = c::NEPSBMinus;
-    for (int r0 = 0; r0 < ne; r0 += Gr) {
-      const int g = lane / N, a = lane - g * N, e = r0 + g;
-      if (g < Gr && e < ne) {
-        const double nu = V[L.nu + e], w = V[L.y + e];
-        const double xq = s_x[a < n ? a : a - n];
-        // partner energy nu x or nu / x (t = nu, m1_opacities.hpp:1547)
-        const double nub = a < n ? mul_rn(nu, xq) : nu / xq;
-        const double wp = nub / T;
-        const double ad = fabs(w - wp);
-        const double xm = w > wp ? w : wp, ym = w < wp ? w : wp;
-        const double sgn = std::signbit(wp - w) ? 1. : -1.;
-        double out[4] = {0., 0., 0., 0.};
-#pragma unroll
-        for (int lep = 0; lep < 2; ++lep) {
-          // NESKernels (lep 0, target e-) / NPSKernels (lep 1, target e+),
-          // kernel_nes.hpp:139-340; couplings (b1, b2) per species, NPS
-          // swaps them
-          const double el = lep == 0 ? eta : -eta;
-          double b1[4], b2[4];
-          b1[0] = lep == 0 ? bpl : bze;  b2[0] = lep == 0 ? bze : bpl;
-          b1[1] = lep == 0 ? bze : bpl;  b2[1] = lep == 0 ? bpl : bze;
-          b1[2] = lep == 0 ? bmi : bze;  b2[2] = lep == 0 ? bze : bmi;
-          b1[3] = lep == 0 ? bze : bmi;  b2[3] = lep == 0 ? bmi : bze;
-          if (ym > el * 1e-3) {
-            // MezzacappaIntOut (kernel_nes.hpp:50-78) with the differences of
-            // ComputeFDIForInelastic (:29-44); F_k(el - w) per energy from LDS
-            double A5[6], C3[6];
-            fdi_bundle<1, 5>(el - wp, A5);
-            fdi_bundle<3, 5>(el - ad, C3);
-            const int fb = (lep == 0 ? L.fym : L.fyq) + e;
-            double fw0 = A5[1] - V[fb], fw1 = A5[2] - V[fb + E], fw2 = A5[3] - V[fb + 2 * E];
-            double fw3 = A5[4] - V[fb + 3 * E], fw4 = A5[5] - V[fb + 4 * E];
-            double fa0 = vFe[lep][3] - C3[3], fa1 = vFe[lep][4] - C3[4];
-            double fa2 = vFe[lep][5] - C3[5];
-            double ws = w + wp;
-            double inv = 1. / (w * w * wp * wp);
-            double q4 = fw3 + 2. * (ws * fw2 + 3. * w * wp * fw1);
-            // the out-kernel is linear in the couplings: b1 U + b2 V
-            double t1 = (sgn * fa2 - fw4) * 0.2;
-            double U = (t1 - ws * q4 + sgn * (xm * fa1 + 2. * (xm * xm) * fa0)) * inv - 6. * fw0;
-            double Vv = (t1 + sgn * (2. * (ym * ym) * fa0 - ym * fa1)) * inv;
-            U *= pre;
-            Vv *= pre;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) out[s] = fma(b1[s], U, fma(b2[s], Vv, out[s]));
-          } else if (xm > el * 1e-3) {
-            // MezzacappaIntOneEnergy (kernel_nes.hpp:82-107)
-            double D3[6];
-            fdi_bundle<0, 2>(el - xm, D3);
-            double f0 = D3[2] - vFe[lep][2], f1 = D3[1], f2 = vFe[lep][1];
-            double f3 = D3[0] + ym * fermi_of_arg(xm - el) / 6.;
-            double f4 = vFe[lep][0] - ym * s_fd0[lep] / 6.;
-            double ix2 = 1. / (xm * xm);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-              double B1 = b1[s], B2 = b2[s];
-              double S = 2. * (B1 + B2) * f0 + (B1 * (ym + 4. * xm) + 3. * B2 * ym) * f1 +
-                         (B1 * (3. * ym - 4. * xm) + B2 * ym) * f2 +
-                         ((B1 + 6. * B2) * ym * ym * 0.2 + B1 * xm * ym + 2. * B1 * xm * xm) * f3 -
-                         ((6. * B1 + B2) * ym * ym * 0.2 - 3. * B1 * xm * ym + 2. * B1 * xm * xm) * f4;
-              out[s] = fma(pre, -sgn * ym * S * ix2, out[s]);
+  for (int64_t chunk = c_begin; chunk < c_end; ++chunk) {
+    const int64_t e0 = chunk * E;
+    const int ne = static_cast<int>(A.nnu - e0 < E ? A.nnu - e0 : E);
+    auto energy = [&](int e) {
+      return A.nu_ld ? A.nu[(e0 + e) * A.nu_ld + p] : A.nu[e0 + e];
+    };
+
+    // ---- Fermi-integral bundles F_1..F_5 at eta - y, eta + y, -eta - y ---------
+    for (int t = lane; t < 3 * ne; t += kWave) {
+      const int e = t / 3, kind = t - 3 * e;
+      const double y = energy(e) / T;
+      double F[6];
+      fdi_bundle<1, 5>(kind == 0 ? eta - y : kind == 1 ? eta + y : -eta - y, F);
+      const int dst = (kind == 0 ? L.fym : kind == 1 ? L.fyp : L.fyq) + e;
+  #pragma unroll
+      for (int k = 1; k <= 5; ++k) V[dst + (k - 1) * E] = F[k];
+    }
+    // ---- distribution values, beta and iso at the energies ----------------------
+    {
+      const PointParams& pp = A.params;
+      for (int t = lane; t < 3 * ne; t += kWave) {
+        {
+          const int e = t / 3, kind = t - 3 * e;
+          const double nu = energy(e);
+          if (kind == 0) {
+            // TotalNuF(nu) of the four species (m1_opacities.hpp:1533-1536)
+            V[L.nu + e] = nu;
+            V[L.y + e] = nu / T;
+            const double lw = log(nu);
+  #pragma unroll
+            for (int s = 0; s < 4; ++s) V[L.gnu + s * E + e] = nuf(nu, lw, Dget(s));
+          } else if (kind == 1) {
+            // AbsOpacity (formulas_beta_iso.hpp:291-319), nue and anue
+            double em[2] = {0., 0.}, ab[2] = {0., 0.};
+            if (A.flags.use_abs_em != 0) {
+              const BetaPoint bp =
+                  beta_point(vin[0], T, vin[2], vin[3], vin[4], vin[5], mu_e, vin[7], vin[8],
+                             pp.use_dU != 0, pp.use_dm_eff != 0, pp.use_WM_ab != 0,
+                             pp.use_decay != 0);
+              beta_em_ab(nu, 0, bp, em[0], ab[0]);
+              beta_em_ab(nu, 1, bp, em[1], ab[1]);
+            }
+  #pragma unroll
+            for (int s = 0; s < 2; ++s) {
+              V[L.bem + s * E + e] = em[s];
+              V[L.bab + s * E + e] = ab[s];
             }
           } else {
-            // MezzacappaIntTwoEnergies (kernel_nes.hpp:111-132)
-            double f0 = s_fd0[lep], f1 = vFe[lep][0], f2 = vFe[lep][1];
-            double ix2 = 1. / (xm * xm);
-            double pol = (11. * ym * ym - 25. * xm * ym + 20. * xm * xm) / 30.;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-              double S = (b1[s] + b2[s]) * (4. * f2 + f0 * pol) + (b1[s] - b2[s]) * (2. * xm - ym) * f1;
-              out[s] = fma(pre, ym * (w - wp) * S * ix2, out[s]);
+            // IsoScattLegCoeff, l = 0 (formulas_beta_iso.hpp:524-560)
+            double iso = 0.;
+            if (A.flags.use_iso != 0) {
+              const double etap = eta_nn_sc(vin[0], T, vin[2]), etan = eta_nn_sc(vin[0], T, vin[3]);
+              double r0p = 1., r0n = 1.;
+              if (pp.use_WM_sc) {
+                r0p = wm_scatt_r0(nu, 1);
+                r0n = wm_scatt_r0(nu, 2);
+              }
+              iso = etap * c::IsoC0p * r0p + etan * c::IsoC0n * r0n;
             }
+            V[L.iso + e] = iso;
           }
         }
-        // detailed balance of the in/out kernels (kernel_nes.hpp:229-234)
-        const double ef = neps_exp_func(wp - w), efx = neps_exp_func(w - wp);
-        // distribution at nubar, formed after the Fermi integrals so that it
-        // is not live through them: production weighted by the occupation,
-        // annihilation by the blocking factor of the same species
-        // (m1_opacities.hpp:1430-1433, 1481-1484)
-        const double lw = log(nub);
-        const double nb2 = nub * nub;
-        const double jac = a < n ? 1. : 1. / (xq * xq);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const double gb = nuf(nub, lw, Dget(s));
-          const double bf = blocking ? 1. - gb : 1.;
-          const double em = -out[s] * efx, ab = out[s] * ef;
-          V[L.sf + s * kWave + lane] = (nb2 * (em * gb)) * jac;
-          V[L.sf + (4 + s) * kWave + lane] = (nb2 * (ab * bf)) * jac;
-        }
       }
-      __syncthreads();
-      reduce_round(r0, L.in, true);
-      __syncthreads();
     }
-  } else {
-    for (int j = lane; j < 8 * E; j += kWave) V[L.in + j] = 0.;
-  }
-  __syncthreads();
+    __syncthreads();
 
-  // ============================ assemble + store ============================
-  // ComputeSpectralOpacitiesNotStimulatedAbs (m1_opacities.hpp:1572-1641);
-  // the stimulated form adds j / c to kappa (:1644-1660)
-  {
-    const double F = c::FourPiHc3, cl = c::Clight, four_pi = 4. * c::Pi;
-    for (int t = lane; t < 16 * ne; t += kWave) {
-      const int f = t / ne, e = t - f * ne;
-      const int s = f & 3, kind = f >> 2;
-      const double nu = V[L.nu + e];
-      const double emb = s < 2 ? V[L.bem + s * E + e] : 0.;
-      const double abb = s < 2 ? V[L.bab + s * E + e] : 0.;
-      const double j = emb + F * (V[L.ip + s * E + e] + V[L.in + s * E + e]);
-      double kap = (abb + F * (V[L.ip + (4 + s) * E + e] + V[L.in + (4 + s) * E + e])) / cl;
-      const double g = V[L.gnu + s * E + e], iso = V[L.iso + e];
-      const double js = four_pi * (nu * nu) * g * iso;
-      double ks = four_pi * (nu * nu) * (1. - g) * iso / cl;
-      if (A.stim == 1) {
-        kap = j / cl + kap;
-        ks = js / cl + ks;
+    // per-energy quadrature sums of one round: lane (g, k) sums integrand k of
+    // energy r0 + g over the first and second half of the nodes in node order
+    // (DoIntegration, integration.hpp:95-108) and scales by t; loads in
+    // batches of four so that their LDS latencies overlap
+    auto reduce_round = [&](int r0, int dst, bool neps) {
+      for (int j = lane; j < 8 * Gr; j += kWave) {
+        const int g = j >> 3, k = j & 7, e = r0 + g;
+        if (e >= ne) continue;
+        const double* f1 = V + L.sf + k * kWave + g * N;
+        const double* f2 = f1 + n;
+        double s1 = 0., s2 = 0.;
+        int i = 0;
+        for (; i + 4 <= n; i += 4) {
+          const double a0 = f1[i], a1 = f1[i + 1], a2 = f1[i + 2], a3 = f1[i + 3];
+          const double b0 = f2[i], b1 = f2[i + 1], b2 = f2[i + 2], b3 = f2[i + 3];
+          const double w0 = s_w[i], w1 = s_w[i + 1], w2 = s_w[i + 2], w3 = s_w[i + 3];
+          s1 = fma(w0, a0, s1);
+          s2 = fma(w0, b0, s2);
+          s1 = fma(w1, a1, s1);
+          s2 = fma(w1, b1, s2);
+          s1 = fma(w2, a2, s1);
+          s2 = fma(w2, b2, s2);
+          s1 = fma(w3, a3, s1);
+          s2 = fma(w3, b3, s2);
+        }
+        for (; i < n; ++i) {
+          s1 = fma(s_w[i], f1[i], s1);
+          s2 = fma(s_w[i], f2[i], s2);
+        }
+        const double t = neps ? V[L.nu + e] : tpair;
+        V[dst + k * E + e] = t * (s1 + s2);
       }
-      const double v = kind == 0 ? j : kind == 1 ? kap : kind == 2 ? js : ks;
-      A.out[(f * A.nnu + e0 + e) * ld + p] = v;
+    };
+
+    // ====================== pair + bremsstrahlung integral ========================
+    if (do_pb) {
+      for (int r0 = 0; r0 < ne; r0 += Gr) {
+        const int g = lane / N, a = lane - g * N, e = r0 + g;
+        if (g < Gr && e < ne) {
+          const double nu = V[L.nu + e], y = V[L.y + e];
+          const double nub = V[L.nub + a], z = V[L.z + a];
+          double em[4] = {0., 0., 0., 0.}, ab[4] = {0., 0., 0., 0.};
+          if (do_pair) {
+            // PairKernels with cos_theta = 1, lmax = 0 (kernel_pair.hpp:135-198)
+            double psi0 = 0., psi1 = 0.;
+            const double emyz = (eta - y) - z;
+            if (!(emyz > 200.)) {
+              // PairPsi (kernel_pair.hpp:16-120)
+              double Fm3[6], Fp3[6];
+              fdi_bundle<3, 5>(emyz, Fm3);
+              fdi_bundle<3, 5>((eta + y) + z, Fp3);
+              const int ey = e, az = a;
+              auto fym = [&](int k) { return V[L.fym + (k - 1) * E + ey]; };
+              auto fyp = [&](int k) { return V[L.fyp + (k - 1) * E + ey]; };
+              auto fzm = [&](int k) { return V[L.fzm + (k - 1) * N + az]; };
+              auto fzp = [&](int k) { return V[L.fzp + (k - 1) * N + az]; };
+              double x0 = 20. * fym(4), x1 = 20. * fzp(4);
+              double x2 = 120. * (fym(2) - fzp(2));
+              double x3 = 40. * (fzp(3) - fym(3));
+              double x4 = 40. * vFe[0][3];
+              double x5 = 40. * Fm3[3] - x4;
+              double x6 = -20. * vFe[0][4];
+              double x7 = 20. * Fm3[4] + x6;
+              double x8 = x4 - 40. * Fp3[3];
+              double x9 = 20. * Fp3[4] + x6;
+              double x10 = 4. * (((((Fm3[5] - fym(5)) - fzm(5)) + fyp(5)) - Fp3[5]) + fzp(5));
+              double x11 = 20. * fyp(4), x12 = 20. * fzm(4);
+              double x13 = 40. * (fyp(3) - fzm(3));
+              double x14 = 120. * (fzm(2) - fyp(2));
+              double yz = y * z;
+              double aux15 = 15. * (yz * yz);
+              psi0 = x10 +
+                     y * (-x0 + x1 + x7 +
+                          y * (x3 + x5 + z * (x2 + z * (120. * (fzp(1) - fym(1))))) +
+                          z * (80. * (fym(3) - fzp(3)) - x2 * z)) +
+                     z * (x0 - x1 + x9 + z * (x3 + x8));
+              psi1 = x10 + y * (-x11 + x12 + x9 + y * (x13 + x8)) +
+                     z * (x11 - x12 + x7 + y * (80. * (fzm(3) - fyp(3)) - x14 * y) +
+                          z * (x13 + x5 + y * (x14 + y * (120. * (fyp(1) - fzm(1))))));
+              psi0 /= aux15;
+              psi1 /= aux15;
+            }
+            // PairPhi: Phi_0 = (a1^2 psi0 + a2^2 psi1) C T^2 / (1 - e^{y+z}),
+            // em = Phi_0 / 2, abs = e^{(w+w')/T} em
+            const double aux = c::PairPhiC * (T * T) / (1. - safe_exp(y + z));
+            const double db = safe_exp((nu + nub) / T);
+            em[0] = 0.5 * ((a1e * psi0 + a2e * psi1) * aux);
+            em[1] = 0.5 * ((a1e * psi1 + a2e * psi0) * aux);
+            em[2] = 0.5 * ((a1x * psi0 + a2x * psi1) * aux);
+            em[3] = 0.5 * ((a1x * psi1 + a2x * psi0) * aux);
+  #pragma unroll
+            for (int s = 0; s < 4; ++s) ab[s] = db * em[s];
+          }
+          double se = 0., sa = 0.;
+          if (do_brem) {
+            if (A.params.brem_impl == 1) {
+              // BremKernelsBRT06 at x = (w + w') / (2T) (kernel_brem_BRT06.hpp:42-69)
+              const double xb = 0.5 * (nu + nub) / T;
+              const double k1x = bessk1_fit(xb) / xb;
+              se = brt_pre * k1x * safe_exp(-xb);
+              sa = brt_pre * k1x * safe_exp(xb);
+            } else {
+              const double x = (nu + nub) / T;
+              sa = hr98_abs(x, hp, s_hc);
+              se = sa * safe_exp(-x);
+            }
+          }
+          // production: partner blocking; annihilation: partner occupation
+          // (SpectralIntegrand, m1_opacities.hpp:1421-1496)
+          const double nb2 = nub * nub;
+          const double jac = a < n ? 1. : 1. / (s_x[a - n] * s_x[a - n]);
+  #pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int ps = s ^ 1;
+            const double pro = (em[s] + se) * V[L.bz + ps * N + a];
+            const double ann = (ab[s] + sa) * V[L.gz + ps * N + a];
+            V[L.sf + s * kWave + lane] = (nb2 * pro) * jac;
+            V[L.sf + (4 + s) * kWave + lane] = (nb2 * ann) * jac;
+          }
+        }
+        __syncthreads();
+        reduce_round(r0, L.ip, false);
+        __syncthreads();
+      }
+    } else {
+      for (int j = lane; j < 8 * E; j += kWave) V[L.ip + j] = 0.;
     }
-  }
+
+    // ================= inelastic scattering on e-+ (NES + NPS) =====================
+    if (do_neps) {
+      const double pre = c::NEPSConst * (T * T);
+      const double bpl = c::NEPSBPlus, bze = c::NEPSBZero, bmi = c::NEPSBMinus;
+      for (int r0 = 0; r0 < ne; r0 += Gr) {
+        const int g = lane / N, a = lane - g * N, e = r0 + g;
+        if (g < Gr && e < ne) {
+          const double nu = V[L.nu + e], w = V[L.y + e];
+          const double xq = s_x[a < n ? a : a - n];
+          // partner energy nu x or nu / x (t = nu, m1_opacities.hpp:1547)
+          const double nub = a < n ? mul_rn(nu, xq) : nu / xq;
+          const double wp = nub / T;
+          const double ad = fabs(w - wp);
+          const double xm = w > wp ? w : wp, ym = w < wp ? w : wp;
+          const double sgn = std::signbit(wp - w) ? 1. : -1.;
+          double out[4] = {0., 0., 0., 0.};
+  #pragma unroll
+          for (int lep = 0; lep < 2; ++lep) {
+            // NESKernels (lep 0, target e-) / NPSKernels (lep 1, target e+),
+            // kernel_nes.hpp:139-340; couplings (b1, b2) per species, NPS
+            // swaps them
+            const double el = lep == 0 ? eta : -eta;
+            double b1[4], b2[4];
+            b1[0] = lep == 0 ? bpl : bze;  b2[0] = lep == 0 ? bze : bpl;
+            b1[1] = lep == 0 ? bze : bpl;  b2[1] = lep == 0 ? bpl : bze;
+            b1[2] = lep == 0 ? bmi : bze;  b2[2] = lep == 0 ? bze : bmi;
+            b1[3] = lep == 0 ? bze : bmi;  b2[3] = lep == 0 ? bmi : bze;
+            if (ym > el * 1e-3) {
+              // MezzacappaIntOut (kernel_nes.hpp:50-78) with the differences of
+              // ComputeFDIForInelastic (:29-44); F_k(el - w) per energy from LDS
+              double A5[6], C3[6];
+              fdi_bundle<1, 5>(el - wp, A5);
+              fdi_bundle<3, 5>(el - ad, C3);
+              const int fb = (lep == 0 ? L.fym : L.fyq) + e;
+              double fw0 = A5[1] - V[fb], fw1 = A5[2] - V[fb + E], fw2 = A5[3] - V[fb + 2 * E];
+              double fw3 = A5[4] - V[fb + 3 * E], fw4 = A5[5] - V[fb + 4 * E];
+              double fa0 = vFe[lep][3] - C3[3], fa1 = vFe[lep][4] - C3[4];
+              double fa2 = vFe[lep][5] - C3[5];
+              double ws = w + wp;
+              double inv = 1. / (w * w * wp * wp);
+              double q4 = fw3 + 2. * (ws * fw2 + 3. * w * wp * fw1);
+              // the out-kernel is linear in the couplings: b1 U + b2 V
+              double t1 = (sgn * fa2 - fw4) * 0.2;
+              double U = (t1 - ws * q4 + sgn * (xm * fa1 + 2. * (xm * xm) * fa0)) * inv - 6. * fw0;
+              double Vv = (t1 + sgn * (2. * (ym * ym) * fa0 - ym * fa1)) * inv;
+              U *= pre;
+              Vv *= pre;
+  #pragma unroll
+              for (int s = 0; s < 4; ++s) out[s] = fma(b1[s], U, fma(b2[s], Vv, out[s]));
+            } else if (xm > el * 1e-3) {
+              // MezzacappaIntOneEnergy (kernel_nes.hpp:82-107)
+              double D3[6];
+              fdi_bundle<0, 2>(el - xm, D3);
+              double f0 = D3[2] - vFe[lep][2], f1 = D3[1], f2 = vFe[lep][1];
+              double f3 = D3[0] + ym * fermi_of_arg(xm - el) / 6.;
+              double f4 = vFe[lep][0] - ym * s_fd0[lep] / 6.;
+              double ix2 = 1. / (xm * xm);
+  #pragma unroll
+              for (int s = 0; s < 4; ++s) {
+                double B1 = b1[s], B2 = b2[s];
+                double S =
+                    2. * (B1 + B2) * f0 + (B1 * (ym + 4. * xm) + 3. * B2 * ym) * f1 +
+                    (B1 * (3. * ym - 4. * xm) + B2 * ym) * f2 +
+                    ((B1 + 6. * B2) * ym * ym * 0.2 + B1 * xm * ym + 2. * B1 * xm * xm) * f3 -
+                    ((6. * B1 + B2) * ym * ym * 0.2 - 3. * B1 * xm * ym + 2. * B1 * xm * xm) * f4;
+                out[s] = fma(pre, -sgn * ym * S * ix2, out[s]);
+              }
+            } else {
+              // MezzacappaIntTwoEnergies (kernel_nes.hpp:111-132)
+              double f0 = s_fd0[lep], f1 = vFe[lep][0], f2 = vFe[lep][1];
+              double ix2 = 1. / (xm * xm);
+              double pol = (11. * ym * ym - 25. * xm * ym + 20. * xm * xm) / 30.;
+  #pragma unroll
+              for (int s = 0; s < 4; ++s) {
+                double S = (b1[s] + b2[s]) * (4. * f2 + f0 * pol) +
+                           (b1[s] - b2[s]) * (2. * xm - ym) * f1;
+                out[s] = fma(pre, ym * (w - wp) * S * ix2, out[s]);
+              }
+            }
+          }
+          // detailed balance of the in/out kernels (kernel_nes.hpp:229-234)
+          double ef, efx;
+          neps_exp_func_pair(wp - w, w - wp, ef, efx);
+          // distribution at nubar, formed after the Fermi integrals so that it
+          // is not live through them: production weighted by the occupation,
+          // annihilation by the blocking factor of the same species
+          // (m1_opacities.hpp:1430-1433, 1481-1484)
+          const double lw = log(nub);
+          const double nb2 = nub * nub;
+          const double jac = a < n ? 1. : 1. / (xq * xq);
+  #pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const double gb = nuf(nub, lw, Dget(s));
+            const double bf = blocking ? 1. - gb : 1.;
+            const double em = -out[s] * efx, ab = out[s] * ef;
+            V[L.sf + s * kWave + lane] = (nb2 * (em * gb)) * jac;
+            V[L.sf + (4 + s) * kWave + lane] = (nb2 * (ab * bf)) * jac;
+          }
+        }
+        __syncthreads();
+        reduce_round(r0, L.in, true);
+        __syncthreads();
+      }
+    } else {
+      for (int j = lane; j < 8 * E; j += kWave) V[L.in + j] = 0.;
+    }
+    __syncthreads();
+
+    // ============================ assemble + store ============================
+    // ComputeSpectralOpacitiesNotStimulatedAbs (m1_opacities.hpp:1572-1641);
+    // the stimulated form adds j / c to kappa (:1644-1660)
+    {
+      const double F = c::FourPiHc3, cl = c::Clight, four_pi = 4. * c::Pi;
+      for (int t = lane; t < 16 * ne; t += kWave) {
+        const int f = t / ne, e = t - f * ne;
+        const int s = f & 3, kind = f >> 2;
+        const double nu = V[L.nu + e];
+        const double emb = s < 2 ? V[L.bem + s * E + e] : 0.;
+        const double abb = s < 2 ? V[L.bab + s * E + e] : 0.;
+        const double j = emb + F * (V[L.ip + s * E + e] + V[L.in + s * E + e]);
+        double kap = (abb + F * (V[L.ip + (4 + s) * E + e] + V[L.in + (4 + s) * E + e])) / cl;
+        const double g = V[L.gnu + s * E + e], iso = V[L.iso + e];
+        const double js = four_pi * (nu * nu) * g * iso;
+        double ks = four_pi * (nu * nu) * (1. - g) * iso / cl;
+        if (A.stim == 1) {
+          kap = j / cl + kap;
+          ks = js / cl + ks;
+        }
+        const double v = kind == 0 ? j : kind == 1 ? kap : kind == 2 ? js : ks;
+        A.out[(f * A.nnu + e0 + e) * ld + p] = v;
+      }
+    }
+    __syncthreads();  // the next chunk rewrites the energy block
+  }  // chunk
 }
 
 }  // namespace
 
-int spectral_energies_per_group(int n) {
+int spectral_energies_per_chunk(int n) {
   const int G = kWave / (2 * n);
   return std::min(4 * G, 16);
+}
+
+int64_t spectral_chunks_per_group(int64_t npts, int64_t nchunk) {
+  // 256 CUs x 12 resident workgroups (3 waves per SIMD); ask for at least 8
+  // workgroups per slot so that the tail of the grid is short
+  constexpr int64_t kWanted = 256 * 12 * 8;
+  const int64_t groups = npts * nchunk;
+  int64_t cpw = groups / kWanted;
+  if (cpw < 1) cpw = 1;
+  if (cpw > nchunk) cpw = nchunk;
+  return cpw;
 }
 
 size_t spectral_lds_bytes(int n, int ech) {
@@ -553,7 +590,7 @@ size_t spectral_lds_bytes(int n, int ech) {
 
 hipError_t launch_spectral_f64(const SpectralKernelArgs& a, hipStream_t stream) {
   if (a.npts <= 0 || a.nnu <= 0) return hipSuccess;
-  const int64_t groups = a.npts * a.nchunk;
+  const int64_t groups = a.npts * ((a.nchunk + a.cpw - 1) / a.cpw);
   dim3 grid(static_cast<unsigned>(groups)), block(kWave);
   hipLaunchKernelGGL(k_spectral_f64, grid, block, spectral_lds_bytes(a.n, a.ech), stream, a);
   return hipGetLastError();

@@ -14,6 +14,8 @@
 //   ComputeM1DensitiesEq             distribution.hpp:448
 //   ComputeM1Opacities               m1_opacities.hpp:1306
 //   ComputeM1OpacitiesNotStimulated  m1_opacities.hpp:1297
+//   ComputeSpectralOpacitiesNotStimulatedAbs  m1_opacities.hpp:1505
+//   ComputeSpectralOpacitiesStimulatedAbs     m1_opacities.hpp:1644
 //
 // Every one of them runs on the GPU through a process-wide default context
 // (device 0, or $BNSAMD_DEVICE).  A per-point call is one small launch; the
@@ -121,6 +123,13 @@ struct M1Opacities {
   BS_REAL kappa_s[total_num_species];
 };
 
+struct SpectralOpacities {  // bns_nurates.hpp:517-523
+  BS_REAL j[total_num_species];        // emissivity
+  BS_REAL kappa[total_num_species];    // absorptivity / inverse mean free path
+  BS_REAL j_s[total_num_species];      // "emissivity" for scattering
+  BS_REAL kappa_s[total_num_species];  // inverse mean free path for scattering
+};
+
 namespace bns_amd {
 
 class Error : public std::runtime_error {
@@ -207,6 +216,14 @@ inline void get_distr(const double* di, size_t i, size_t n, NuDistributionParams
   }
 }
 
+inline void get_spectral(const double* o, size_t i, size_t n, SpectralOpacities& r) {
+  for (int s = 0; s < 4; ++s) {
+    r.j[s] = o[(BNSAMD_SP_J + s) * n + i];
+    r.kappa[s] = o[(BNSAMD_SP_KAPPA + s) * n + i];
+    r.j_s[s] = o[(BNSAMD_SP_J_S + s) * n + i];
+    r.kappa_s[s] = o[(BNSAMD_SP_KAPPA_S + s) * n + i];
+  }
+}
 inline void get_out(const double* o, size_t i, size_t n, M1Opacities& r) {
   for (int s = 0; s < total_num_species; ++s) {
     r.eta_0[s] = o[(BNSAMD_OUT_ETA_0 + s) * n + i];
@@ -284,6 +301,52 @@ class Context {
       check(bnsamd_m1_opacities_host(ctx_, int64_t(n), hp.data(), hd.data(), &f, &p,
                                      stimulated ? 1 : 0, precision, ho.data()));
       for (size_t k = 0; k < n; ++k) detail::get_out(ho.data(), k, n, out[idx[k]]);
+    }
+  }
+
+  // Spectral opacities of npts points at nnu energies, device SoA buffers,
+  // asynchronous on the context stream (bnsamd_spectral_opacities).
+  void ComputeSpectralOpacitiesDevice(int64_t npts, int64_t ld, const double* d_pts,
+                                      const double* d_distr, int64_t nnu, const double* d_nu,
+                                      int64_t nu_ld, const OpacityFlags& flags,
+                                      const OpacityParams& params, bool stimulated,
+                                      double* d_out) {
+    bnsamd_flags f = detail::to_c(flags);
+    bnsamd_params p = detail::to_c(params);
+    check(bnsamd_spectral_opacities(ctx_, npts, ld, d_pts, d_distr, nnu, d_nu, nu_ld, &f, &p,
+                                    stimulated ? 1 : 0, d_out));
+  }
+
+  // Host batch in the reference's types: entry i is point pts[i] at energy
+  // nu[i] [MeV], with its own distr_pars, opacity_flags and opacity_pars, as
+  // the reference's per-(energy, point) calls (tests/tests.hpp:1174-1176).
+  void ComputeSpectralOpacities(const MyQuadrature* quad_1d, const BS_REAL* nu,
+                                const GreyOpacityParams* pts, size_t npts, SpectralOpacities* out,
+                                bool stimulated) {
+    if (npts == 0) return;
+    if (!nu || !pts || !out) throw Error(BNSAMD_ERR_INVALID, "null energy, point or output array");
+    SetQuadrature(quad_1d, quad_1d);
+    std::map<decltype(detail::key_of(pts[0])), std::vector<size_t>> groups;
+    for (size_t i = 0; i < npts; ++i) groups[detail::key_of(pts[i])].push_back(i);
+    for (const auto& kv : groups) {
+      const std::vector<size_t>& idx = kv.second;
+      const size_t n = idx.size();
+      std::vector<double> hp(n * BNSAMD_PT_NFIELDS), hd(n * BNSAMD_DI_NFIELDS), hn(n),
+          ho(n * BNSAMD_SP_NFIELDS);
+      for (size_t k = 0; k < n; ++k) {
+        const GreyOpacityParams& g = pts[idx[k]];
+        detail::put_eos(g.eos_pars, k, n, hp.data());
+        detail::put_m1(g.m1_pars, k, n, hp.data());
+        detail::put_distr(g.distr_pars, k, n, hd.data());
+        hn[k] = nu[idx[k]];
+      }
+      const GreyOpacityParams& g0 = pts[idx[0]];
+      bnsamd_flags f = detail::to_c(g0.opacity_flags);
+      bnsamd_params p = detail::to_c(g0.opacity_pars);
+      // one energy per point: nnu = 1, energies per point
+      check(bnsamd_spectral_opacities_host(ctx_, int64_t(n), hp.data(), hd.data(), 1, hn.data(),
+                                           1, &f, &p, stimulated ? 1 : 0, ho.data()));
+      for (size_t k = 0; k < n; ++k) detail::get_spectral(ho.data(), k, n, out[idx[k]]);
     }
   }
 
@@ -403,6 +466,39 @@ inline void ComputeM1OpacitiesNotStimulated(const MyQuadrature* quad_1d,
                                             const GreyOpacityParams* pts, size_t npts,
                                             M1Opacities* out) {
   bns_amd::DefaultContext().ComputeM1Opacities(quad_1d, quad_2d, pts, npts, out, false);
+}
+
+// ComputeSpectralOpacitiesNotStimulatedAbs (m1_opacities.hpp:1505-1641): one
+// point at energy nu [MeV], the distribution as given in distr_pars
+inline SpectralOpacities ComputeSpectralOpacitiesNotStimulatedAbs(
+    const BS_REAL nu, MyQuadrature* quad_1d, GreyOpacityParams* my_grey_opacity_params) {
+  SpectralOpacities r;
+  bns_amd::DefaultContext().ComputeSpectralOpacities(quad_1d, &nu, my_grey_opacity_params, 1,
+                                                     &r, false);
+  return r;
+}
+
+// ComputeSpectralOpacitiesStimulatedAbs (m1_opacities.hpp:1644-1660): one point
+inline SpectralOpacities ComputeSpectralOpacitiesStimulatedAbs(
+    const BS_REAL nu, MyQuadrature* quad_1d, GreyOpacityParams* my_grey_opacity_params) {
+  SpectralOpacities r;
+  bns_amd::DefaultContext().ComputeSpectralOpacities(quad_1d, &nu, my_grey_opacity_params, 1,
+                                                     &r, true);
+  return r;
+}
+
+// Batched forms: entry i is point pts[i] at energy nu[i].
+inline void ComputeSpectralOpacitiesNotStimulatedAbs(const BS_REAL* nu,
+                                                     const MyQuadrature* quad_1d,
+                                                     const GreyOpacityParams* pts, size_t npts,
+                                                     SpectralOpacities* out) {
+  bns_amd::DefaultContext().ComputeSpectralOpacities(quad_1d, nu, pts, npts, out, false);
+}
+
+inline void ComputeSpectralOpacitiesStimulatedAbs(const BS_REAL* nu, const MyQuadrature* quad_1d,
+                                                  const GreyOpacityParams* pts, size_t npts,
+                                                  SpectralOpacities* out) {
+  bns_amd::DefaultContext().ComputeSpectralOpacities(quad_1d, nu, pts, npts, out, true);
 }
 
 #endif  // BNS_NURATES_AMD_HPP_

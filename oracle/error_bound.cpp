@@ -158,39 +158,40 @@ int orc_bound_spectral_opacities(int64_t np, int n, const double* x, const doubl
   Flags fl{flags[0], flags[1], flags[2], flags[3], flags[4]};
   Params pr{params[0], params[1], params[2], params[3], params[4], params[5], params[6], params[7]};
   if (nthreads <= 0) nthreads = omp_get_max_threads();
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4) collapse(2)
+  if (n > kSpecMaxNodes / 2) return 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
   for (int64_t i = 0; i < np; ++i) {
-    for (int64_t e = 0; e < nnu; ++e) {
-      Eos<Mag> eos{pts[P_NB * np + i], pts[P_T * np + i], pts[P_YP * np + i],
-                   pts[P_YN * np + i], pts[P_MUP * np + i], pts[P_MUN * np + i],
-                   pts[P_MUE * np + i], pts[P_DU * np + i], pts[P_DMEFF * np + i]};
-      Distr<Mag> d;
-      if (distr) {
-        for (int s = 0; s < 4; ++s) {
-          d.w_t[s] = distr[(D_WT + s) * np + i];
-          d.temp_t[s] = distr[(D_TT + s) * np + i];
-          d.eta_t[s] = distr[(D_ETAT + s) * np + i];
-          d.w_f[s] = distr[(D_WF + s) * np + i];
-          d.temp_f[s] = distr[(D_TF + s) * np + i];
-          d.c_f[s] = distr[(D_CF + s) * np + i];
-          d.beta_f[s] = distr[(D_BF + s) * np + i];
-        }
-      } else {
-        M1<Mag> m;
-        for (int s = 0; s < 4; ++s) {
-          m.n[s] = pts[(P_N0 + s) * np + i];
-          m.J[s] = pts[(P_J0 + s) * np + i];
-          m.chi[s] = pts[(P_CHI0 + s) * np + i];
-        }
-        distr_from_m1(m, d);
+    Eos<Mag> eos{pts[P_NB * np + i], pts[P_T * np + i], pts[P_YP * np + i],
+                 pts[P_YN * np + i], pts[P_MUP * np + i], pts[P_MUN * np + i],
+                 pts[P_MUE * np + i], pts[P_DU * np + i], pts[P_DMEFF * np + i]};
+    Distr<Mag> d;
+    if (distr) {
+      for (int s = 0; s < 4; ++s) {
+        d.w_t[s] = distr[(D_WT + s) * np + i];
+        d.temp_t[s] = distr[(D_TT + s) * np + i];
+        d.eta_t[s] = distr[(D_ETAT + s) * np + i];
+        d.w_f[s] = distr[(D_WF + s) * np + i];
+        d.temp_f[s] = distr[(D_TF + s) * np + i];
+        d.c_f[s] = distr[(D_CF + s) * np + i];
+        d.beta_f[s] = distr[(D_BF + s) * np + i];
       }
-      Mag o[16];
-      spectral_point(q, eos, d, fl, pr, Mag(nu[nu_ld ? e * nu_ld + i : e]), stim, o);
-      for (int f = 0; f < 16; ++f) {
-        out[(f * nnu + e) * np + i] = o[f].v;
-        mag[(f * nnu + e) * np + i] = o[f].m;
+    } else {
+      M1<Mag> m;
+      for (int s = 0; s < 4; ++s) {
+        m.n[s] = pts[(P_N0 + s) * np + i];
+        m.J[s] = pts[(P_J0 + s) * np + i];
+        m.chi[s] = pts[(P_CHI0 + s) * np + i];
       }
+      distr_from_m1(m, d);
     }
+    spectral_point(
+        q, eos, d, fl, pr, nnu, [&](int64_t e) { return Mag(nu[nu_ld ? e * nu_ld + i : e]); },
+        stim, [&](int64_t e, const Mag* o) {
+          for (int f = 0; f < 16; ++f) {
+            out[(f * nnu + e) * np + i] = o[f].v;
+            mag[(f * nnu + e) * np + i] = o[f].m;
+          }
+        });
   }
   return 0;
 }

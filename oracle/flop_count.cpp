@@ -105,6 +105,7 @@ int orc_count_spectral_opacities(int64_t np, int n, const double* x, const doubl
                                  const double* pts, int64_t nnu, const double* nu,
                                  const int* flags, const int* params, int stim, double* out,
                                  int64_t* counts) {
+  if (n < 1 || n > kSpecMaxNodes / 2) return 1;
   g_t = Tally{};
   Quad q{n, x, w};
   Flags fl{flags[0], flags[1], flags[2], flags[3], flags[4]};
@@ -121,11 +122,11 @@ int orc_count_spectral_opacities(int64_t np, int n, const double* x, const doubl
     }
     Distr<Cnt> d;
     distr_from_m1(m, d);
-    for (int64_t k = 0; k < nnu; ++k) {
-      Cnt o[16];
-      spectral_point(q, e, d, fl, pr, Cnt(nu[k]), stim, o);
-      for (int f = 0; f < 16; ++f) out[(f * nnu + k) * np + i] = o[f].v;
-    }
+    spectral_point(
+        q, e, d, fl, pr, nnu, [&](int64_t k) { return Cnt(nu[k]); }, stim,
+        [&](int64_t k, const Cnt* o) {
+          for (int f = 0; f < 16; ++f) out[(f * nnu + k) * np + i] = o[f].v;
+        });
   }
   counts[0] = g_t.add;
   counts[1] = g_t.mul;

@@ -297,17 +297,18 @@ int orc_spectral_opacities(int64_t np, int n, const double* x, const double* w,
   Flags fl = flags_from(flags);
   Params pr = params_from(params);
   if (nthreads <= 0) nthreads = omp_get_max_threads();
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4) collapse(2)
+  if (n > kSpecMaxNodes / 2) return 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
   for (int64_t i = 0; i < np; ++i) {
-    for (int64_t e = 0; e < nnu; ++e) {
-      Eos<double> eos = eos_at(pts, np, i);
-      Distr<double> d;
-      if (distr) distr_load(distr, np, i, d);
-      else distr_from_m1(m1_at(pts, np, i), d);
-      double o[16];
-      spectral_point(q, eos, d, fl, pr, nu[nu_ld ? e * nu_ld + i : e], stim, o);
-      for (int f = 0; f < 16; ++f) out[(f * nnu + e) * np + i] = o[f];
-    }
+    Eos<double> eos = eos_at(pts, np, i);
+    Distr<double> d;
+    if (distr) distr_load(distr, np, i, d);
+    else distr_from_m1(m1_at(pts, np, i), d);
+    spectral_point(
+        q, eos, d, fl, pr, nnu, [&](int64_t e) { return nu[nu_ld ? e * nu_ld + i : e]; }, stim,
+        [&](int64_t e, const double* o) {
+          for (int f = 0; f < 16; ++f) out[(f * nnu + e) * np + i] = o[f];
+        });
   }
   return 0;
 }

@@ -1462,9 +1462,14 @@ inline void m1_opacities_point(const Quad& q1, const Quad& q2, const Eos<R>& eos
 }
 
 // ---------------------------------------------------------------------------
-// Spectral opacities at one neutrino energy nu
-// (ComputeSpectralOpacitiesNotStimulatedAbs / ...StimulatedAbs,
-// m1_opacities.hpp:1505-1660).
+// Spectral opacities (ComputeSpectralOpacitiesNotStimulatedAbs /
+// ...StimulatedAbs, m1_opacities.hpp:1505-1660), reuse-aware: the values
+// that depend on the point alone (F_k(+-eta), the pair/brem partner nodes
+// with their distribution values and F_1..F_5(eta -+ z)) are evaluated once
+// per point, those that depend on the energy alone (F_1..F_5(eta -+ y),
+// F_1..F_5(-eta - y)) once per energy, each by the arithmetic the reference
+// repeats for every call of SpectralIntegrand -- which it makes eight times
+// per node, once per integral (integration.hpp:511-546).  No bit changes.
 // ---------------------------------------------------------------------------
 // Per-point NES/NPS scalars (the F_k(+-eta) that NESKernels / NPSKernels
 // evaluate at every call, kernel_nes.hpp:139-340)
@@ -1481,20 +1486,56 @@ inline void neps_point(const Eos<R>& eos, NepsPoint<R>& P) {
   }
 }
 
-// SpectralIntegrand (m1_opacities.hpp:1316-1501) at the partner energy nub:
-// f[s] = nub^2 x production, f[4 + s] = nub^2 x annihilation, for the
-// reactions switched on in fl (kirchoff_flag is false, :1451-1464).
+constexpr int kSpecMaxNodes = 64;  // 2n, n <= 32
+
+// Point-level tables of the spectral path
 template <class R>
-inline void spectral_integrand(R nu, R nub, const Eos<R>& eos, const Distr<R>& d,
-                               const Flags& fl, const Params& pr, const NepsPoint<R>& P,
-                               R f[8]) {
-  const R T = eos.T;
-  R gb[4];
-  for (int s = 0; s < 4; ++s) gb[s] = total_nuf(nub, d, s);
-  KOut<R> pk, bk, ik;
-  for (int s = 0; s < 4; ++s) {
-    pk.em[s] = pk.abs[s] = bk.em[s] = bk.abs[s] = ik.em[s] = ik.abs[s] = R(0.);
+struct SpecPoint {
+  NepsPoint<R> P;
+  R tpair;                          // s_pair = 0.5 * 4.364 T (m1_opacities.hpp:1546)
+  R nub[kSpecMaxNodes];             // pair/brem partner nodes t x_i, t / x_i
+  R gb[4][kSpecMaxNodes];           // TotalNuF at the nodes
+  R Fzm[6][kSpecMaxNodes];          // F_1..F_5(eta - z), z = nub / T
+  R Fzp[6][kSpecMaxNodes];          // F_1..F_5(eta + z)
+};
+
+template <class R>
+inline void spectral_point_tables(const Quad& q, const Eos<R>& eos, const Distr<R>& d,
+                                  const Flags& fl, SpecPoint<R>& S) {
+  neps_point(eos, S.P);
+  const int n = q.n;
+  S.tpair = R(0.5 * 4.364) * eos.T;
+  for (int i = 0; i < n; ++i) {
+    S.nub[i] = S.tpair * R(q.x[i]);      // integration.hpp:525
+    S.nub[n + i] = S.tpair / R(q.x[i]);  // :529
   }
+  if (!(fl.use_pair || fl.use_brem)) return;
+  const R eta = eos.mu_e / eos.T;
+  for (int a = 0; a < 2 * n; ++a) {
+    for (int s = 0; s < 4; ++s) S.gb[s][a] = total_nuf(S.nub[a], d, s);
+    if (fl.use_pair) {
+      const R z = S.nub[a] / eos.T;
+      R b[6];
+      fdi_bundle(1, eta - z, b);
+      for (int kk = 1; kk <= 5; ++kk) S.Fzm[kk][a] = b[kk];
+      fdi_bundle(1, eta + z, b);
+      for (int kk = 1; kk <= 5; ++kk) S.Fzp[kk][a] = b[kk];
+    }
+  }
+}
+
+// SpectralIntegrand (m1_opacities.hpp:1316-1501) of the pair + brem
+// integral (inelastic scattering off, :1520-1521) at partner node a:
+// f[s] = nub^2 x production, f[4 + s] = nub^2 x annihilation.  Fy[0], Fy[1]
+// hold F_1..F_5 at eta - y and eta + y, y = nu / T.
+template <class R>
+inline void spectral_integrand_pb(R nu, int a, const SpecPoint<R>& S, const R Fy[2][6],
+                                  const Eos<R>& eos, const Flags& fl, const Params& pr,
+                                  R f[8]) {
+  const R T = eos.T;
+  const R nub = S.nub[a];
+  KOut<R> pk, bk;
+  for (int s = 0; s < 4; ++s) pk.em[s] = pk.abs[s] = bk.em[s] = bk.abs[s] = R(0.);
   if (fl.use_pair) {
     // PairKernels with cos_theta = 1, lmax = 0 (kernel_pair.hpp:165-198)
     const R eta = eos.mu_e / T;
@@ -1503,15 +1544,14 @@ inline void spectral_integrand(R nu, R nub, const Eos<R>& eos, const Distr<R>& d
     for (int kk = 0; kk < 6; ++kk)
       for (int m = 0; m < 7; ++m) F[kk][m] = R(0.);
     if (!(eta - y - z > R(200.))) {
-      R b[6];
-      const R args[4] = {eta - y, eta - z, eta + y, eta + z};
-      const int slot[4] = {A_EMY, A_EMZ, A_EPY, A_EPZ};
-      for (int m = 0; m < 4; ++m) {
-        fdi_bundle(1, args[m], b);
-        for (int kk = 1; kk <= 5; ++kk) F[kk][slot[m]] = b[kk];
+      for (int kk = 1; kk <= 5; ++kk) {
+        F[kk][A_EMY] = Fy[0][kk];
+        F[kk][A_EPY] = Fy[1][kk];
+        F[kk][A_EMZ] = S.Fzm[kk][a];
+        F[kk][A_EPZ] = S.Fzp[kk][a];
       }
-      fdi_bundle(3, eta, b);
-      for (int kk = 3; kk <= 5; ++kk) F[kk][A_E] = b[kk];
+      for (int kk = 3; kk <= 5; ++kk) F[kk][A_E] = S.P.Fe[0][kk];
+      R b[6];
       fdi_bundle(3, eta - y - z, b);
       for (int kk = 3; kk <= 5; ++kk) F[kk][A_EMYZ] = b[kk];
       fdi_bundle(3, eta + y + z, b);
@@ -1530,43 +1570,57 @@ inline void spectral_integrand(R nu, R nub, const Eos<R>& eos, const Distr<R>& d
       bk.abs[s] = sa;
     }
   }
-  if (fl.use_inelastic_scatt) {
-    R G[2][6][3];
-    neps_fdi_table(nu / T, nub / T, P, G);
-    inel_kernels(nu, nub, P, G, false, ik);
-  }
-  R bf[4];
-  for (int s = 0; s < 4; ++s) bf[s] = pr.neglect_blocking ? R(1.) : R(1.) - gb[s];
-  R pro[4], ann[4];
-  for (int s = 0; s < 4; ++s) pro[s] = (pk.em[s] + bk.em[s]) * bf[partner(s)];
-  for (int s = 0; s < 4; ++s) pro[s] += ik.em[s] * gb[s];
-  for (int s = 0; s < 4; ++s) ann[s] = (pk.abs[s] + bk.abs[s]) * gb[partner(s)];
-  for (int s = 0; s < 4; ++s) ann[s] += ik.abs[s] * bf[s];
   const R nb2 = nub * nub;
   for (int s = 0; s < 4; ++s) {
-    f[s] = nb2 * pro[s];
-    f[4 + s] = nb2 * ann[s];
+    const int ps = partner(s);
+    const R bf = pr.neglect_blocking ? R(1.) : R(1.) - S.gb[ps][a];
+    f[s] = nb2 * ((pk.em[s] + bk.em[s]) * bf);
+    f[4 + s] = nb2 * ((pk.abs[s] + bk.abs[s]) * S.gb[ps][a]);
+  }
+}
+
+// SpectralIntegrand of the NEPS integral (only inelastic scattering on,
+// :1564-1566) at partner energy nub; Fw[lep] holds F_1..F_5 at eta_lep - w,
+// w = nu / T (the NESKernels / NPSKernels value ComputeFDIForInelastic forms
+// at every call, kernel_nes.hpp:29-44).
+template <class R>
+inline void spectral_integrand_neps(R nu, R nub, const Distr<R>& d, const Params& pr,
+                                    const NepsPoint<R>& P, const R Fw[2][6], R f[8]) {
+  const R w = nu / P.T, wp = nub / P.T;
+  R G[2][6][3];
+  const R ad = rfabs(w - wp);
+  const R ym = (w < wp) ? w : wp;
+  for (int lep = 0; lep < 2; ++lep) {
+    const R eta = P.eta[lep];
+    for (int kk = 0; kk < 6; ++kk) G[lep][kk][0] = G[lep][kk][1] = G[lep][kk][2] = R(0.);
+    if (!(ym > eta * R(1e-3))) continue;  // only the full branch reads them
+    R b0[6], b2[6];
+    fdi_bundle(1, eta - wp, b0);
+    fdi_bundle(3, eta - ad, b2);
+    for (int kk = 1; kk <= 5; ++kk) {
+      G[lep][kk][0] = b0[kk];
+      G[lep][kk][1] = Fw[lep][kk];
+      G[lep][kk][2] = kk >= 3 ? b2[kk] : R(0.);
+    }
+  }
+  KOut<R> ik;
+  inel_kernels(nu, nub, P, G, false, ik);
+  const R nb2 = nub * nub;
+  for (int s = 0; s < 4; ++s) {
+    const R gb = total_nuf(nub, d, s);
+    const R bf = pr.neglect_blocking ? R(1.) : R(1.) - gb;
+    f[s] = nb2 * (ik.em[s] * gb);
+    f[4 + s] = nb2 * (ik.abs[s] * bf);
   }
 }
 
 // GaussLegendreIntegrate1D (integration.hpp:511-546) of the eight spectral
-// integrands over (0, inf) with scale t: nodes t x_i and t / x_i, the second
-// half weighted by 1 / x_i^2, each half summed in node order
-// (DoIntegration, integration.hpp:97-108).
+// integrands over (0, inf) with scale t: f1 at the nodes t x_i, f2 at t / x_i
+// divided by x_i^2, each half summed in node order (DoIntegration,
+// integration.hpp:97-108), result t (sum1 + sum2).
 template <class R>
-inline void spectral_integrate(const Quad& q, R t, R nu, const Eos<R>& eos, const Distr<R>& d,
-                               const Flags& fl, const Params& pr, const NepsPoint<R>& P,
-                               R res[8]) {
+inline void spectral_sums(const Quad& q, R t, const R* f1, const R* f2, R res[8]) {
   const int n = q.n;
-  std::vector<R> f1(size_t(8) * n), f2(size_t(8) * n);
-  for (int i = 0; i < n; ++i) {
-    const R xi = R(q.x[i]);
-    R v[8];
-    spectral_integrand(nu, t * xi, eos, d, fl, pr, P, v);
-    for (int kk = 0; kk < 8; ++kk) f1[size_t(kk) * n + i] = v[kk];
-    spectral_integrand(nu, t / xi, eos, d, fl, pr, P, v);
-    for (int kk = 0; kk < 8; ++kk) f2[size_t(kk) * n + i] = v[kk] / (xi * xi);
-  }
   for (int kk = 0; kk < 8; ++kk) {
     R s1 = R(0.), s2 = R(0.);
     for (int i = 0; i < n; ++i) s1 += R(q.w[i]) * f1[size_t(kk) * n + i];
@@ -1576,26 +1630,52 @@ inline void spectral_integrate(const Quad& q, R t, R nu, const Eos<R>& eos, cons
 }
 
 // ComputeSpectralOpacitiesNotStimulatedAbs (m1_opacities.hpp:1505-1641) and,
-// with stim = 1, ComputeSpectralOpacitiesStimulatedAbs (:1644-1660).
+// with stim = 1, ComputeSpectralOpacitiesStimulatedAbs (:1644-1660), at one
+// energy nu of a point whose tables S are built.
 // out[S_J + s], out[S_KAPPA + s], out[S_JS + s], out[S_KAPPAS + s].
 template <class R>
-inline void spectral_point(const Quad& q, const Eos<R>& eos, const Distr<R>& d,
-                           const Flags& fl, const Params& pr, R nu, int stim, R out[16]) {
-  NepsPoint<R> P;
-  neps_point(eos, P);
+inline void spectral_energy(const Quad& q, const Eos<R>& eos, const Distr<R>& d,
+                            const Flags& fl, const Params& pr, const SpecPoint<R>& S, R nu,
+                            int stim, R out[16]) {
+  const int n = q.n;
   R g_nu[4];
   for (int s = 0; s < 4; ++s) g_nu[s] = total_nuf(nu, d, s);
-  // pair + bremsstrahlung with the NEPS flag cleared, scale 2.182 T
-  // (:1519-1560); NEPS alone, scale nu (:1562-1570)
-  const R s_pair = R(0.5 * 4.364) * eos.T;
-  Flags f1 = fl;
-  f1.use_inelastic_scatt = 0;
-  R Ip[8], In[8];
-  spectral_integrate(q, s_pair, nu, eos, d, f1, pr, P, Ip);
+  // F_1..F_5 at eta - y, eta + y (pair) and -eta - w (positron NEPS); the
+  // electron NEPS value at eta - w is the first of them (w = y = nu / T)
+  R Fy[2][6], Fw[2][6];
+  for (int kk = 0; kk < 6; ++kk) Fy[0][kk] = Fy[1][kk] = Fw[0][kk] = Fw[1][kk] = R(0.);
+  const R y = nu / eos.T;
+  if (fl.use_pair || fl.use_inelastic_scatt == 1) fdi_bundle(1, S.P.eta[0] - y, Fy[0]);
+  if (fl.use_pair) fdi_bundle(1, S.P.eta[0] + y, Fy[1]);
+  if (fl.use_inelastic_scatt == 1) fdi_bundle(1, S.P.eta[1] - y, Fw[1]);
+  for (int kk = 0; kk < 6; ++kk) Fw[0][kk] = Fy[0][kk];
+  std::vector<R> f1(size_t(8) * n), f2(size_t(8) * n);
+  R v[8];
+  // pair + bremsstrahlung, scale 2.182 T (:1519-1560)
+  R Ip[8];
+  for (int kk = 0; kk < 8; ++kk) Ip[kk] = R(0.);
+  if (fl.use_pair || fl.use_brem) {
+    for (int i = 0; i < n; ++i) {
+      const R xi = R(q.x[i]);
+      spectral_integrand_pb(nu, i, S, Fy, eos, fl, pr, v);
+      for (int kk = 0; kk < 8; ++kk) f1[size_t(kk) * n + i] = v[kk];
+      spectral_integrand_pb(nu, n + i, S, Fy, eos, fl, pr, v);
+      for (int kk = 0; kk < 8; ++kk) f2[size_t(kk) * n + i] = v[kk] / (xi * xi);
+    }
+    spectral_sums(q, S.tpair, f1.data(), f2.data(), Ip);
+  }
+  // NEPS alone, scale nu (:1562-1570)
+  R In[8];
   for (int kk = 0; kk < 8; ++kk) In[kk] = R(0.);
   if (fl.use_inelastic_scatt == 1) {
-    Flags f2{0, 0, 0, 1, 0};
-    spectral_integrate(q, nu, nu, eos, d, f2, pr, P, In);
+    for (int i = 0; i < n; ++i) {
+      const R xi = R(q.x[i]);
+      spectral_integrand_neps(nu, nu * xi, d, pr, S.P, Fw, v);
+      for (int kk = 0; kk < 8; ++kk) f1[size_t(kk) * n + i] = v[kk];
+      spectral_integrand_neps(nu, nu / xi, d, pr, S.P, Fw, v);
+      for (int kk = 0; kk < 8; ++kk) f2[size_t(kk) * n + i] = v[kk] / (xi * xi);
+    }
+    spectral_sums(q, nu, f1.data(), f2.data(), In);
   }
   R em[4], ab[4];
   for (int s = 0; s < 4; ++s) em[s] = ab[s] = R(0.);
@@ -1621,6 +1701,21 @@ inline void spectral_point(const Quad& q, const Eos<R>& eos, const Distr<R>& d,
       out[S_KAPPA + s] = out[S_J + s] / c + out[S_KAPPA + s];
       out[S_KAPPAS + s] = out[S_JS + s] / c + out[S_KAPPAS + s];
     }
+  }
+}
+
+// All energies of one point: nu(e) for e < nnu, out16(e) the 16 outputs of
+// energy e.
+template <class R, class NuF, class OutF>
+inline void spectral_point(const Quad& q, const Eos<R>& eos, const Distr<R>& d,
+                           const Flags& fl, const Params& pr, int64_t nnu, NuF nu, int stim,
+                           OutF out16) {
+  SpecPoint<R> S;
+  spectral_point_tables(q, eos, d, fl, S);
+  for (int64_t e = 0; e < nnu; ++e) {
+    R o[16];
+    spectral_energy(q, eos, d, fl, pr, S, nu(e), stim, o);
+    out16(e, o);
   }
 }
 

@@ -19,6 +19,7 @@ from bns_nurates_amd import workloads as W
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "cpp", "m1_host_api_check")
+SPEC_DRIVER = os.path.join(ROOT, "tests", "cpp", "spectral_host_api_check")
 
 NAMES_SRC = r"""
 #include "bns_nurates_amd.hpp"
@@ -48,6 +49,13 @@ int main() {
   void (*eq)(const MyEOSParams*, const NuDistributionParams*, M1Quantities*) =
       &ComputeM1DensitiesEq;
   (void)fp; (void)gl; (void)eq;
+  static_assert(std::is_same<decltype(ComputeSpectralOpacitiesNotStimulatedAbs(10., &q, &g)),
+                             SpectralOpacities>::value, "");
+  SpectralOpacities (*sp)(const BS_REAL, MyQuadrature*, GreyOpacityParams*) =
+      &ComputeSpectralOpacitiesStimulatedAbs;
+  (void)sp;
+  SpectralOpacities so{};
+  (void)(so.j[0] + so.kappa[0] + so.j_s[0] + so.kappa_s[0]);
   M1Opacities o{};
   return int(o.eta_0[0] + o.kappa_0_a[0] + o.eta[0] + o.kappa_a[0] + o.kappa_s[total_num_species - 1]);
 }
@@ -101,3 +109,45 @@ def test_cpp_driver_matches_reference_outputs(golden, tmp_path):
     assert O.parity_excess(distr, d_ref, d_mag).max() <= 1.0
     _, m = O.bound_m1_opacities(pts, 10, g["flags"], g["params"], int(g["stim"]))
     assert check_parity("C++ host API, 4096 pts brt06", batch, ref, m, W.OUT_NAMES) <= 1.0
+
+
+@pytest.mark.gpu
+def test_cpp_spectral_driver_matches_reference_outputs(golden, tmp_path):
+    """tests/cpp/spectral_host_api_check makes the reference spectral test's
+    calls (six points, equilibrium distributions from NuEquilibriumParams,
+    100 energies, one ComputeSpectralOpacitiesNotStimulatedAbs per (energy,
+    point)) through the C++ API, then the same pairs batched."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    assert os.path.exists(SPEC_DRIVER), "tests/cpp/spectral_host_api_check not built (run build())"
+    from tests.gpu_util import check_parity, say
+    g = golden("spectral_test6_n10_hr98.npz")
+    pts, nu, ref = g["pts"], g["nu"], g["out"]
+    npts, nnu = pts.shape[1], nu.shape[0]
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        f.write(np.array([npts, nnu], np.int64).tobytes())
+        f.write(np.array([int(g["nquad"]), int(g["stim"])], np.int32).tobytes())
+        f.write(np.asarray(g["flags"], np.int32).tobytes())
+        f.write(np.asarray(g["params"], np.int32).tobytes())
+        f.write(np.ascontiguousarray(pts, np.float64).tobytes())
+        f.write(np.ascontiguousarray(nu, np.float64).tobytes())
+    say(f"C++ spectral driver: {npts} points x {nnu} energies")
+    r = subprocess.run([SPEC_DRIVER, str(inp), str(outp)], capture_output=True, text=True,
+                       timeout=120)
+    say(r.stdout.strip(), r.stderr.strip())
+    assert r.returncode == 0
+    raw = np.fromfile(outp, dtype=np.float64)
+    k = 16 * nnu * npts
+    single = raw[:k].reshape(16, nnu, npts)
+    batch = raw[k:2 * k].reshape(16, nnu, npts)
+    distr = raw[2 * k:].reshape(28, npts)
+    assert np.array_equal(single, batch, equal_nan=True)
+    # the equilibrium distributions the driver made, against the reference's
+    np.testing.assert_allclose(distr, g["distr"], rtol=1e-14, atol=0)
+    _, m = O.bound_spectral_opacities(pts, nu, int(g["nquad"]), g["flags"], g["params"],
+                                      stim=int(g["stim"]), distr=g["distr"])
+    names = [f"{q}[{s}]" for q in ("j", "kappa", "j_s", "kappa_s") for s in range(4)]
+    assert check_parity("C++ host API, spectral test6", batch.reshape(16, -1),
+                        ref.reshape(16, -1), m.reshape(16, -1), names) <= 1.0
