@@ -262,6 +262,66 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def config0_eval(ctx, steps=20, repeats=5):
+    """The reference's own CPU-runnable case (4096 seeded points, n = 10,
+    BRT06, all reactions): GPU kernel time, and the genuine reference
+    (oracle/_ref, compiled from the reference's headers, OpenMP over points)
+    on the host cores over the same points, with the parity of the two."""
+    import torch
+    import oracle as O
+    from bns_nurates_amd import workloads as W
+    cfg = W.WORKLOADS["pts4096_n10_brt06"]
+    pts = W.make_points(cfg["npts"], seed=cfg["seed"])
+    prm = W.params_default_all(cfg["brem"])
+    npts = pts.shape[1]
+    ctx.set_quadrature(cfg["nquad"])
+    dp = torch.from_numpy(pts).cuda()
+    out = torch.empty((20, npts), dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        ctx.compute_m1_opacities(dp, flags=W.FLAGS_ALL, params=prm, out=out)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        ctx.compute_m1_opacities(dp, flags=W.FLAGS_ALL, params=prm, out=out)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    got = out.cpu().numpy()
+    res = {"workload": "pts4096_n10_brt06: 4096 seeded points (seed 0), n=10, all reactions, "
+                       "BRT06, default_all, distributions reconstructed from M1",
+           "value": npts * steps / wall, "unit": "points/s", "kernel_ms": kern_ms,
+           "note": "4096 workgroups do not fill the 256 CUs; the grid workload is the "
+                   "throughput figure"}
+    if O.available("ref"):
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        O.m1_opacities(pts[:, :256], cfg["nquad"], W.FLAGS_ALL, prm, kind="ref", nthreads=threads)
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            ref = O.m1_opacities(pts, cfg["nquad"], W.FLAGS_ALL, prm, kind="ref",
+                                 nthreads=threads)
+            ts.append(time.perf_counter() - t0)
+        _, mag = O.bound_m1_opacities(pts, cfg["nquad"], W.FLAGS_ALL, prm)
+        res["cpu_reference"] = {
+            "value": npts / float(np.median(ts)), "unit": "points/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"all {npts} points, median of {repeats} passes; the reference's own "
+                      "ComputeM1Opacities compiled from its headers (oracle/Makefile, -O2 "
+                      "-ffp-contract=off, OpenMP over points)"}
+        res["gpu_kernel_over_reference"] = (npts / (kern_ms * 1e-3)) / res["cpu_reference"]["value"]
+        res["parity_vs_reference"] = {"bar_excess_max": float(O.parity_excess(got, ref, mag).max())}
+        log(f"config 0: GPU {npts / (kern_ms * 1e-3):.4g} points/s (kernel), reference "
+            f"{res['cpu_reference']['value']:.4g} points/s on {threads} threads")
+    else:
+        res["cpu_reference"] = None
+        log("config 0: oracle/_ref not built here; reference not timed")
+    return res
+
+
 def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0):
     """The spectral leg: events-timed passes of the spectral kernel over the
     spectral workload (inputs resident in HBM), an oracle parity sample, and
@@ -515,6 +575,8 @@ def main():
             "speedup_vs_fp64": res[brem]["kernel_ms"] / h["kernel_ms"] if brem in res else None,
             "fp32_parts": "distribution values (exp/log), HR98 channel functions",
             "rel_err_vs_fp64_oracle": acc}
+    if not args.no_cpu:
+        line["config0"] = config0_eval(ctx)
     if not args.no_spectral:
         line["spectral"] = spectral_eval(ctx, max(3, args.steps // 2), 1,
                                          cpu=not args.no_cpu)

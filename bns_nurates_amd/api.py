@@ -145,6 +145,15 @@ def load_library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise BnsAmdError(f"{LIB_PATH} is not built: run make -C bns_nurates_amd/csrc")
+    # One HIP runtime per process.  PyTorch's ROCm wheels bundle their own
+    # libamdhip64 and load it by file name; loaded before torch, this
+    # library would bring in the system copy (same soname) and torch would
+    # then load a second runtime that cannot share the device with the
+    # first.  Importing torch first makes this library bind to torch's copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     vp, i64, dp = C.c_void_p, C.c_int64, C.POINTER(C.c_double)
     sig = {
@@ -168,6 +177,10 @@ def load_library():
                                             vp], C.c_int),
         "bnsamd_distr_from_m1": ([vp, i64, i64, vp, vp], C.c_int),
         "bnsamd_equilibrium": ([vp, i64, i64, vp, vp], C.c_int),
+        "bnsamd_m1_densities_eq": ([vp, i64, i64, vp, vp], C.c_int),
+        "bnsamd_distr_from_m1_host": ([vp, i64, vp, vp], C.c_int),
+        "bnsamd_equilibrium_host": ([vp, i64, vp, vp], C.c_int),
+        "bnsamd_m1_densities_eq_host": ([vp, i64, vp, vp], C.c_int),
         "bnsamd_fdi": ([vp, C.c_int, i64, vp, vp], C.c_int),
         "bnsamd_server_create": ([C.POINTER(_ServerConfig), C.POINTER(vp)], C.c_int),
         "bnsamd_server_destroy": ([vp], None),

@@ -61,3 +61,14 @@ def test_flag_and_param_coercion():
     assert api.as_flags(None) == api.OpacityFlags()
     with pytest.raises(api.BnsAmdError):
         api.as_params(np.zeros(3))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbnsamd.so not built")
+def test_python_mirror_declares_every_signature():
+    """Every C entry point has ctypes argtypes/restype in api.load_library:
+    an undeclared one would receive Python ints as 32-bit C ints and
+    truncate pointers."""
+    from bns_nurates_amd import api
+    lib = api.load_library()
+    missing = [n for n in sorted(declared_symbols()) if getattr(lib, n).argtypes is None]
+    assert not missing, missing
