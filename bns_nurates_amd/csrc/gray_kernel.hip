@@ -94,6 +94,8 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     return r;
   };
   const volatile double (*vFe)[6] = s_Fe;
+  // anux with the distribution of nux: its TotalNuF values are copied
+  const bool same23 = __builtin_amdgcn_readfirstlane(spec_same(Dget(2), Dget(3)) ? 1 : 0) != 0;
 
   double acc[16];  // [0..3] n_em, [4..7] n_ab, [8..11] e_em, [12..15] e_ab
 #pragma unroll
@@ -524,25 +526,35 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       double gn[4], gb[4], on[4], ob[4];  // f and 1 - f at nu and nubar
       if constexpr (MIX) {
         float l1 = logf(static_cast<float>(nu)), l2 = logf(static_cast<float>(nub));
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        auto species = [&](int s) {
           Spec d = Dget(s);
           GOmg a = nuf_mixed(nu, l1, d), b = nuf_mixed(nub, l2, d);
           gn[s] = a.g;
           on[s] = a.omg;
           gb[s] = b.g;
           ob[s] = b.omg;
-        }
+        };
+#pragma unroll
+        for (int s = 0; s < 3; ++s) species(s);
+        if (!same23) species(3);
       } else {
         double l1 = log(nu), l2 = log(nub);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        auto species = [&](int s) {
           Spec d = Dget(s);
           gn[s] = nuf(nu, l1, d);
           gb[s] = nuf(nub, l2, d);
           on[s] = 1. - gn[s];
           ob[s] = 1. - gb[s];
-        }
+        };
+#pragma unroll
+        for (int s = 0; s < 3; ++s) species(s);
+        if (!same23) species(3);
+      }
+      if (same23) {
+        gn[3] = gn[2];
+        gb[3] = gb[2];
+        on[3] = on[2];
+        ob[3] = ob[2];
       }
       // in/out kernels for (nu, nubar) and the swapped pair: the swapped
       // kernel is the negated out-integral with NEPSExpFunc arguments

@@ -250,6 +250,16 @@ __device__ __forceinline__ double eta_nn_sc(double nb, double T, double yN) {
   return nN * aux / sqrt(1. + aux * aux);
 }
 
+// Two species with bitwise identical distribution parameters have identical
+// TotalNuF values at every energy (callers commonly give anux the moments of
+// nux, as the reference's drivers do, tests/tests.hpp:551-553).  The
+// comparison is made once per point; the result is wave-uniform.
+__device__ __forceinline__ bool spec_same(const Spec& a, const Spec& b) {
+  auto eq = [](double x, double y) { return __double_as_longlong(x) == __double_as_longlong(y); };
+  return eq(a.wt, b.wt) && eq(a.mut, b.mut) && eq(a.invTt, b.invTt) && eq(a.wf, b.wf) &&
+         eq(a.bf, b.bf) && eq(a.cf, b.cf) && eq(a.invTf, b.invTf) && eq(a.lbf, b.lbf);
+}
+
 // NuDistributionParams fields of one species (bns_nurates.hpp:387-401) as
 // the caller gives them
 __device__ __forceinline__ Spec spec_from_fields(double w_t, double temp_t, double eta_t,

@@ -198,6 +198,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     return r;
   };
   const volatile double (*vFe)[6] = s_Fe;
+  // anux with the distribution of nux: its TotalNuF values are copied
+  const bool same23 = __builtin_amdgcn_readfirstlane(spec_same(Dget(2), Dget(3)) ? 1 : 0) != 0;
   const double tpair = c::PairScale * T;  // s_pair = 0.5 * 4.364 T (m1_opacities.hpp:1546)
   // partner node a of the pair/brem integral: t x_a or t / x_{a-n}
   // (integration.hpp:525-534), formed with the reference's rounding
@@ -520,12 +522,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
           const double lw = log(nub);
           const double nb2 = nub * nub;
           const double jac = a < n ? 1. : 1. / (xq * xq);
-  #pragma unroll
+          double gb[4];
+#pragma unroll
+          for (int s = 0; s < 3; ++s) gb[s] = nuf(nub, lw, Dget(s));
+          gb[3] = same23 ? gb[2] : nuf(nub, lw, Dget(3));
+#pragma unroll
           for (int s = 0; s < 4; ++s) {
-            const double gb = nuf(nub, lw, Dget(s));
-            const double bf = blocking ? 1. - gb : 1.;
+            const double bf = blocking ? 1. - gb[s] : 1.;
             const double em = -out[s] * efx, ab = out[s] * ef;
-            V[L.sf + s * kWave + lane] = (nb2 * (em * gb)) * jac;
+            V[L.sf + s * kWave + lane] = (nb2 * (em * gb[s])) * jac;
             V[L.sf + (4 + s) * kWave + lane] = (nb2 * (ab * bf)) * jac;
           }
         }

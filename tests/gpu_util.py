@@ -57,3 +57,17 @@ def check_parity(tag, got, ref, mag, names=None, rtol=1e-12, k=64.0):
         say(f"   worst excess at {nm} point {loc[-1]}: got {np.asarray(got)[loc]!r} "
             f"ref {np.asarray(ref)[loc]!r} m {np.asarray(mag)[loc]!r}")
     return float(ex.max())
+
+
+def distinct_heavy_species(pts, seed):
+    """A copy of a point block whose anux has its own moments (the workloads
+    give anux those of nux; the kernels copy the nux distribution values when
+    the two agree, so this exercises the other branch)."""
+    from bns_nurates_amd import workloads as W
+    p = np.array(pts, copy=True)
+    rng = np.random.default_rng(seed)
+    npts = p.shape[1]
+    p[W.P_N0 + 3] *= np.exp(rng.uniform(-1.0, 1.0, npts))
+    p[W.P_J0 + 3] *= np.exp(rng.uniform(-1.0, 1.0, npts))
+    p[W.P_CHI0 + 3] = rng.uniform(1.0 / 3.0, 1.0, npts)
+    return p

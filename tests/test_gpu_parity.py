@@ -17,7 +17,7 @@ import pytest
 
 import oracle as O
 from bns_nurates_amd import workloads as W
-from tests.gpu_util import check_parity, rel_err, report, say
+from tests.gpu_util import check_parity, distinct_heavy_species, rel_err, report, say
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -126,6 +126,26 @@ def test_n16_vs_oracle(ctx, brem):
     ref, m = O.bound_m1_opacities(pts, 16, W.FLAGS_ALL, prm)
     out = run(ctx, pts, 16, W.FLAGS_ALL, prm)
     assert check_parity(f"n16 brem{brem}", out, ref, m, W.OUT_NAMES) <= 1.0
+
+
+def test_distinct_heavy_species_vs_oracle(ctx):
+    pts = distinct_heavy_species(W.make_points(384, seed=5), seed=6)
+    prm = W.params_default_all(W.BREM_BRT06)
+    prm[7] = 0  # blocking on: the distribution enters every reaction
+    say("anux moments distinct from nux: 384 points, n=16, FP64 and mixed")
+    ref, m = O.bound_m1_opacities(pts, 16, W.FLAGS_ALL, prm)
+    out = run(ctx, pts, 16, W.FLAGS_ALL, prm)
+    assert check_parity("distinct anux", out, ref, m, W.OUT_NAMES) <= 1.0
+    mixed = run(ctx, pts, 16, W.FLAGS_ALL, prm, precision=1)
+    assert check_parity("distinct anux, mixed", mixed, ref, m, W.OUT_NAMES, rtol=1e-5) <= 1.0
+    # the nux-shared path on the same points gives the nux / anux outputs the
+    # oracle gives when anux copies nux
+    same = np.array(pts, copy=True)
+    for f in (W.P_N0, W.P_J0, W.P_CHI0):
+        same[f + 3] = same[f + 2]
+    ref2, m2 = O.bound_m1_opacities(same, 16, W.FLAGS_ALL, prm)
+    assert check_parity("anux = nux", run(ctx, same, 16, W.FLAGS_ALL, prm), ref2, m2,
+                        W.OUT_NAMES) <= 1.0
 
 
 def test_mixed_orders_and_blocking_vs_oracle(ctx):

@@ -16,7 +16,7 @@ import pytest
 
 import oracle as O
 from bns_nurates_amd import workloads as W
-from tests.gpu_util import check_parity, say
+from tests.gpu_util import check_parity, distinct_heavy_species, say
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -104,6 +104,19 @@ def test_large_orders_vs_oracle(ctx, nquad):
         ref, m = O.bound_spectral_opacities(pts, nu, nquad, W.FLAGS_ALL, prm, stim=1)
         worst = max(worst, parity(f"n{nquad} brem{brem}", out, ref, m))
     assert worst <= 1.0
+
+
+def test_distinct_heavy_species_vs_oracle(ctx):
+    """anux with its own moments (the kernel copies the nux distribution
+    values when the two agree)."""
+    pts = distinct_heavy_species(W.make_points(96, seed=15), seed=16)
+    nu = np.exp(np.linspace(np.log(0.3), np.log(300.0), 9))
+    prm = W.params_default_all(W.BREM_HR98)
+    prm[7] = 0
+    say("spectral, anux moments distinct from nux: 96 points x 9 energies, n=10")
+    out = run(ctx, pts, nu, 10, W.FLAGS_ALL, prm, stim=1)
+    ref, m = O.bound_spectral_opacities(pts, nu, 10, W.FLAGS_ALL, prm, stim=1)
+    assert parity("distinct anux", out, ref, m) <= 1.0
 
 
 def test_host_entry_matches_device(ctx):
