@@ -139,3 +139,23 @@ def test_reference_bindings_script_flow():
         worst = max(worst, check_parity(f"gray part {part}", got_g, g_ref, g_mag))
         assert all(v >= 0 for k in spec for v in spec[k])
     assert worst <= 1.0
+
+
+def test_thc_export(tmp_path):
+    """The thorn export writes the headers and the library, and a thorn
+    source that includes the exported header compiles with the reference's
+    names; a second export writes nothing."""
+    import subprocess
+    from bns_nurates_amd import export_thc
+    dest = tmp_path / "src"
+    dest.mkdir()
+    first = export_thc.export(str(dest), log=lambda *a: None)
+    assert sorted(first) == ["bns_nurates_amd.h", "bns_nurates_amd.hpp", "libbnsamd.so"]
+    assert export_thc.export(str(dest), log=lambda *a: None) == []
+    src = dest / "weakrates.cpp"
+    src.write_text('#include "bns_nurates_amd.hpp"\n'
+                   "M1Opacities f(MyQuadrature* q, GreyOpacityParams* g) {\n"
+                   "  return ComputeM1Opacities(q, q, g);\n}\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", str(dest), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
