@@ -93,6 +93,40 @@ def mixed_error_stats(got, ref, mag):
             "fp64_determined_frac": det_frac}
 
 
+def reference_float_comparison(ctx, sp, params, nquad=10):
+    """The reference's own all-float build (REAL_TYPE=float, oracle/_ref)
+    and the mixed form on the same points at n = 10 (the reference's largest
+    order), both against the FP64 oracle: the upper bound of all-FP32 error
+    the mixed form is set against."""
+    import torch
+    import oracle as O
+    from bns_nurates_amd import api
+    from bns_nurates_amd import workloads as W
+    ref, mag = O.bound_m1_opacities(sp, nquad, W.FLAGS_ALL, params)
+    f32 = O.m1_opacities(sp, nquad, W.FLAGS_ALL, params, kind="ref_f32")
+    ctx.set_quadrature(nquad)
+    mix = ctx.compute_m1_opacities(torch.from_numpy(sp).cuda(), flags=W.FLAGS_ALL, params=params,
+                                   precision=api.MIXED)
+    torch.cuda.synchronize()
+    mix = mix.cpu().numpy()
+
+    def summary(got):
+        g, f = got[MIXED_ROWS], ref[MIXED_ROWS]
+        det = (64 * O.U * np.abs(mag[MIXED_ROWS]) <= 1e-7 * np.abs(f)) & (f != 0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            e = np.abs(g - f) / np.abs(f)
+        e = e[det]
+        fin = e[np.isfinite(e)]
+        ex = O.parity_excess(got, ref, mag, rtol=MIXED_RTOL)[MIXED_ROWS]
+        # the maximum over finite errors; non-finite outputs are counted apart
+        return {"rel_err_max_finite": float(f"{fin.max():.3g}"),
+                "rel_err_median": float(f"{np.median(fin):.3g}"),
+                "nonfinite_frac": float(f"{(~np.isfinite(g)).mean():.3g}"),
+                "bar_1e-5_fail_frac": float(f"{(ex > 1).mean():.3g}")}
+    return {"sample": f"first {sp.shape[1]} points of the grid workload, n={nquad}, BRT06",
+            "reference_all_float": summary(f32), "mixed": summary(mix)}
+
+
 # request mix of the serving workload: meshblock sizes and weights, share of
 # single-point calls, quadrature orders, precisions
 SERVE_SIZES = [8 ** 3, 16 ** 3, 32 ** 3, 64 ** 3]
@@ -575,6 +609,9 @@ def main():
             "speedup_vs_fp64": res[brem]["kernel_ms"] / h["kernel_ms"] if brem in res else None,
             "fp32_parts": "distribution values (exp/log), HR98 channel functions",
             "rel_err_vs_fp64_oracle": acc}
+        if brem == "brt06" and O.available("ref_f32"):
+            line["mixed"]["vs_reference_all_float"] = reference_float_comparison(
+                ctx, np.ascontiguousarray(pts[:, :2048]), params)
     if not args.no_cpu:
         line["config0"] = config0_eval(ctx)
     if not args.no_spectral:

@@ -304,37 +304,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     }
     __syncthreads();
 
-    // per-energy quadrature sums of one round: lane (g, k) sums integrand k of
-    // energy r0 + g over the first and second half of the nodes in node order
-    // (DoIntegration, integration.hpp:95-108) and scales by t; loads in
-    // batches of four so that their LDS latencies overlap
+    // per-energy quadrature sums of one round (GaussLegendreIntegrate1D,
+    // integration.hpp:511-546): lane (g, k, h) sums integrand k of energy
+    // r0 + g over half h of the nodes in node order (DoIntegration,
+    // integration.hpp:95-108), loads in batches of four so that their LDS
+    // latencies overlap; the neighbouring lane holds the other half and lane
+    // h = 0 stores t (s1 + s2).  16 Gr <= 64 for n >= 4; the partner lanes
+    // j and j ^ 1 are in the same iteration for any n.
     auto reduce_round = [&](int r0, int dst, bool neps) {
-      for (int j = lane; j < 8 * Gr; j += kWave) {
-        const int g = j >> 3, k = j & 7, e = r0 + g;
-        if (e >= ne) continue;
-        const double* f1 = V + L.sf + k * kWave + g * N;
-        const double* f2 = f1 + n;
-        double s1 = 0., s2 = 0.;
-        int i = 0;
-        for (; i + 4 <= n; i += 4) {
-          const double a0 = f1[i], a1 = f1[i + 1], a2 = f1[i + 2], a3 = f1[i + 3];
-          const double b0 = f2[i], b1 = f2[i + 1], b2 = f2[i + 2], b3 = f2[i + 3];
-          const double w0 = s_w[i], w1 = s_w[i + 1], w2 = s_w[i + 2], w3 = s_w[i + 3];
-          s1 = fma(w0, a0, s1);
-          s2 = fma(w0, b0, s2);
-          s1 = fma(w1, a1, s1);
-          s2 = fma(w1, b1, s2);
-          s1 = fma(w2, a2, s1);
-          s2 = fma(w2, b2, s2);
-          s1 = fma(w3, a3, s1);
-          s2 = fma(w3, b3, s2);
+      for (int j0 = 0; j0 < 16 * Gr; j0 += kWave) {
+        const int j = j0 + lane;
+        const int g = j >> 4, k = (j >> 1) & 7, h = j & 1, e = r0 + g;
+        const bool live = j < 16 * Gr && e < ne;
+        double s = 0.;
+        if (live) {
+          const double* f = V + L.sf + k * kWave + g * N + h * n;
+          int i = 0;
+          for (; i + 4 <= n; i += 4) {
+            const double a0 = f[i], a1 = f[i + 1], a2 = f[i + 2], a3 = f[i + 3];
+            s = fma(s_w[i], a0, s);
+            s = fma(s_w[i + 1], a1, s);
+            s = fma(s_w[i + 2], a2, s);
+            s = fma(s_w[i + 3], a3, s);
+          }
+          for (; i < n; ++i) s = fma(s_w[i], f[i], s);
         }
-        for (; i < n; ++i) {
-          s1 = fma(s_w[i], f1[i], s1);
-          s2 = fma(s_w[i], f2[i], s2);
+        const double other = __shfl_xor(s, 1, kWave);  // every lane takes part
+        if (live && h == 0) {
+          const double t = neps ? V[L.nu + e] : tpair;
+          V[dst + k * E + e] = t * (s + other);
         }
-        const double t = neps ? V[L.nu + e] : tpair;
-        V[dst + k * E + e] = t * (s1 + s2);
       }
     };
 

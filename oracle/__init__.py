@@ -29,6 +29,9 @@ _LIBS = {
     "bound": os.path.join(HERE, "liborc_bound.so"),
     "fast": os.path.join(HERE, "liborc_fast.so"),
     "ref": os.path.join(HERE, "_ref", "libbnsref.so"),
+    # the reference's own all-float build (REAL_TYPE=float, bns_nurates.hpp:29-34):
+    # an upper bound of all-FP32 error, for comparison with the mixed form
+    "ref_f32": os.path.join(HERE, "_ref", "libbnsref_f32.so"),
 }
 _cache = {}
 
@@ -56,7 +59,7 @@ def load(kind: str = "orc"):
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
     lib = C.CDLL(path)
-    p = "ref" if kind == "ref" else "orc"
+    p = "ref" if kind in ("ref", "ref_f32") else "orc"
     if kind == "bound":
         f = lib.orc_bound_m1_opacities
         f.argtypes = [_i64, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_void_p, _ip, _ip,

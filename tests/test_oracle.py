@@ -346,3 +346,19 @@ def test_flop_counts_scale_with_order():
     f16 = sum(c16[k] for k in ("add", "mul", "div", "sqrt"))
     # the work is dominated by the O(n^2) cells: 16^2 / 10^2 = 2.56
     assert 2.0 < f16 / f10 < 3.0
+
+
+@pytest.mark.skipif(not O.available("ref_f32"), reason="reference float build not made here")
+def test_reference_float_build_is_the_float_path():
+    """oracle/_ref/libbnsref_f32.so is the reference compiled with its
+    REAL_TYPE=float switch (bns_nurates.hpp:29-34): float accuracy on the
+    well-conditioned outputs, not the double results."""
+    pts = W.make_points(64, seed=3)
+    prm = W.params_default_all(W.BREM_BRT06)
+    f32 = O.m1_opacities(pts, 8, W.FLAGS_ALL, prm, kind="ref_f32")
+    ref, mag = O.bound_m1_opacities(pts, 8, W.FLAGS_ALL, prm)
+    well = (np.abs(mag) <= 1e3 * np.abs(ref)) & (ref != 0) & np.isfinite(f32)
+    rel = np.abs(f32[well] - ref[well]) / np.abs(ref[well])
+    assert well.sum() > 100
+    assert np.median(rel) < 1e-4 and np.median(rel) > 1e-9
+    assert not np.array_equal(f32, ref)
