@@ -26,6 +26,8 @@ namespace bns {
 // The body of both kernel variants: MIX = false is the FP64 path, MIX = true
 // the mixed-precision path (FP32 distribution values and HR98 channel
 // functions, everything else FP64).
+constexpr int kInvT = 63;  // slot of 1 / T in the per-point LDS block
+
 template <bool MIX>
 __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
@@ -35,7 +37,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_y[64];      // node energies / T, formed as the reference does
   // wave-uniform per-point data, read back with broadcast LDS loads so that
   // it occupies neither scalar nor vector registers through the cell loops
-  __shared__ double s_in[64];     // the 21 inputs (+ 28 distribution values)
+  __shared__ double s_in[64];     // the 21 inputs (+ 28 distribution values), 1 / T
   __shared__ Spec s_D[4];
   __shared__ double s_Fe[2][6], s_fd0[2];
   __shared__ HrChan s_hc[3];
@@ -65,6 +67,8 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     d.lbf = MIX ? log(d.bf) : 0.;
     s_D[s] = d;
   }
+  // 1 / T: energies / T are formed as div_by_recip(nu, T, 1 / T)
+  if (lane == 63) s_in[kInvT] = 1. / T;
 
   // ---- per-point constants --------------------------------------------------
   const double eta = mu_e / T;
@@ -196,11 +200,11 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     // (m1_opacities.hpp:783-787, kernel_pair.hpp:142-143) so that thresholds
     // such as eta - y - z > 200 decide as they do there.
     const double tpair = c::PairScale * T;
-    auto node_y = [&](int a) {
-      double xa = tb[L.nd + ND_REC * a + ND_X];
-      double nua = tb[L.nd + ND_REC * a + ND_INV] != 0. ? tpair / xa : mul_rn(tpair, xa);
-      return nua / T;
+    auto node_nu = [&](int a) {
+      const double* nd = tb + L.nd + ND_REC * a;
+      return nd[ND_INV] != 0. ? div_by_recip(tpair, nd[ND_X], nd[ND_RX]) : mul_rn(tpair, nd[ND_X]);
     };
+    auto node_y = [&](int a) { return div_by_recip(node_nu(a), T, vin[kInvT]); };
     for (int task = lane; task < 3 * N; task += kWave) {
       if (task < 2 * N) {
         if (do_pair) {
@@ -218,9 +222,8 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
         }
       } else {
         int a = task - 2 * N;
-        double xa = tb[L.nd + ND_REC * a + ND_X];
-        double nu = tb[L.nd + ND_REC * a + ND_INV] != 0. ? tpair / xa : mul_rn(tpair, xa);
-        s_y[a] = nu / T;
+        double nu = node_nu(a);
+        s_y[a] = div_by_recip(nu, T, vin[kInvT]);
         double lw = MIX ? 0. : log(nu);
         float lw32 = MIX ? logf(static_cast<float>(nu)) : 0.f;
 #pragma unroll
@@ -311,7 +314,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           psi1 = x10 + y * (-x11 + x12 + x9 + y * (x13 + x8)) +
                  z * (x11 - x12 + x7 + y * (80. * (F3emz - F3epy) - x14 * y) +
                       z * (x13 + x5 + y * (x14 + y * (120. * (F1epy - F1emz)))));
-          double ia = 1. / aux;
+          double ia = div_normal(1., aux);
           psi0 *= ia;
           psi1 *= ia;
         }
@@ -452,10 +455,11 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
         nu = mul_rn(hx, hr[NP_OMX]);
         nub = mul_rn(hx, hr[NP_OPX]);
       } else {
-        nu = mul_rn(ht, hr[NP_OMX]) / xi;
-        nub = mul_rn(ht, hr[NP_OPX]) / xi;
+        nu = div_by_recip(mul_rn(ht, hr[NP_OMX]), xi, hr[NP_RXI]);
+        nub = div_by_recip(mul_rn(ht, hr[NP_OPX]), xi, hr[NP_RXI]);
       }
-      double w = nu / T, wp = nub / T;
+      const double rT = vin[kInvT];
+      double w = div_by_recip(nu, T, rT), wp = div_by_recip(nub, T, rT);
       double ad = fabs(w - wp);
       double xm = wp, ym = w;  // w < w' on the whole grid
       if (w > wp) { xm = w; ym = wp; }

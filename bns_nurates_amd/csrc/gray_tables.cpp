@@ -47,7 +47,7 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
     W[n + i] = w2[i] / (x2[i] * x2[i]);
   }
   for (int a = 0; a < N; ++a) {
-    block[L.nd + ND_REC * a + ND_Y] = y[a];
+    block[L.nd + ND_REC * a + ND_RX] = 1.0 / x2[a < n ? a : a - n];
     block[L.nd + ND_REC * a + ND_WQ] = W[a] * y[a] * y[a];
     block[L.nd + ND_REC * a + ND_X] = x2[a < n ? a : a - n];
     block[L.nd + ND_REC * a + ND_INV] = a < n ? 0.0 : 1.0;
@@ -115,7 +115,7 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
         double* r = rec[h].data();
         r[NP_W] = w;
         r[NP_WP] = wp;
-        r[NP_AD] = std::fabs(w - wp);
+        r[NP_RXI] = 1.0 / xi;
         r[NP_EF] = neps_exp_func(wp - w);
         r[NP_EFX] = neps_exp_func(w - wp);
         r[NP_INVDEN] = 1.0 / (w * w * wp * wp);

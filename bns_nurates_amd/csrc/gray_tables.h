@@ -20,9 +20,9 @@ constexpr int kMaxQuad = 32;  // quadrature order n <= 32 (2n = 64 nodes)
 
 // 1D quadrature record
 enum { Q1_X = 0, Q1_W, Q1_REC = 2 };
-// 2D node record: y = nu / T, number weight times y^2, the quadrature node x
-// and whether the energy is t x (0) or t / x (1)
-enum { ND_Y = 0, ND_WQ, ND_X, ND_INV, ND_REC = 4 };
+// 2D node record: 1 / x correctly rounded, number weight times y^2 (y =
+// nu / T), the quadrature node x and whether the energy is t x (0) or t / x (1)
+enum { ND_RX = 0, ND_WQ, ND_X, ND_INV, ND_REC = 4 };
 // pair/brem cell record (a <= b), cells sorted by S = y_a + y_b
 enum {
   CL_AB = 0,  // a * 256 + b
@@ -44,7 +44,7 @@ enum {
 enum {
   NP_W = 0,   // nu / T
   NP_WP,      // nubar / T
-  NP_AD,      // |w - w'|
+  NP_RXI,     // 1 / x_i correctly rounded
   NP_EF,      // NEPSExpFunc(w' - w)
   NP_EFX,     // NEPSExpFunc(w - w')
   NP_INVDEN,  // 1 / (w^2 w'^2)
