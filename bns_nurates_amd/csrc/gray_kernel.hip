@@ -26,8 +26,6 @@ namespace bns {
 // The body of both kernel variants: MIX = false is the FP64 path, MIX = true
 // the mixed-precision path (FP32 distribution values and HR98 channel
 // functions, everything else FP64).
-constexpr int kInvT = 63;  // slot of 1 / T in the per-point LDS block
-
 template <bool MIX>
 __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
@@ -380,7 +378,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
                         (x * x + 2. * x * y + 5. * (y * y) / 3. + 1.);
           double bb = c::Pi2OneEighth + x + y;
           double b2 = bb * bb;
-          double snd = snum / (c::SqrtPi + b2 * b2);
+          double snd = div_normal(snum, c::SqrtPi + b2 * b2);
           double lsnd = log(snd), lsdx = cr[CL_LSD];
           double pnd = exp(-hr_p * lsnd);
           double px11 = cr[CL_X11], px093 = cr[CL_X093];
@@ -393,13 +391,14 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
             const volatile HrChan& h = s_hc[ch];
             double sd = exp(-hr_p * (h.lsd_pt + lsdx));
             double sbase = exp(hr_pinv * log(pnd + sd));
-            double fb = 1. + 1. / (fx * h.f_pt);
+            // the operands are normal and the quotients O(1e-30..1e30)
+            double fb = 1. + div_normal(1., fx * h.f_pt);
             double gbm = 1. - px11 * h.g0;
-            double cb = 1.1 * px11 * h.h / (2.3 + h.h * px093 + 0.0001 * px12) * 30. /
-                        (30. + 0.005 * px28);
+            double cb = div_normal(1.1 * px11 * h.h, 2.3 + h.h * px093 + 0.0001 * px12);
+            cb = div_normal(cb * 30., 30. + 0.005 * px28);
             double sb = sbase * fb * (1. + cb * gbm);
             double gx = h.gamma_half_gb;
-            double sabs = h.gamma_over_T / (x * x + gx * gx) * sb;
+            double sabs = div_normal(h.gamma_over_T, x * x + gx * gx) * sb;
             tot += h.weight * sabs;
           }
           bab = 3. * c::BremConst * tot * hr_medium;

@@ -86,7 +86,7 @@ __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
                       (xc * xc + 2. * xc * y + 5. * (y * y) / 3. + 1.);
   const double bb = c::Pi2OneEighth + xc + y;
   const double b2 = bb * bb;
-  const double snd = snum / (c::SqrtPi + b2 * b2);
+  const double snd = div_normal(snum, c::SqrtPi + b2 * b2);
   const double lx = log(xc);
   const double px11 = exp(1.1 * lx), pxm4 = exp(-4. * lx), px093 = exp(0.93 * lx);
   const double px12 = exp(1.2 * lx), px28 = exp(2.8 * lx);
@@ -102,11 +102,12 @@ __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
     const double gx = hc[ch].gamma_half_gb, got = hc[ch].gamma_over_T, wt = hc[ch].weight;
     double sd = exp(-hp.p * (lsd_pt + lsdx));
     double sbase = exp(hp.pinv * log(pnd + sd));
-    double fb = 1. + 1. / (fx * f_pt);
+    double fb = 1. + div_normal(1., fx * f_pt);
     double gbm = 1. - px11 * g0;
-    double cb = 1.1 * px11 * h / (2.3 + h * px093 + 0.0001 * px12) * 30. / (30. + 0.005 * px28);
+    double cb = div_normal(div_normal(1.1 * px11 * h, 2.3 + h * px093 + 0.0001 * px12) * 30.,
+                           30. + 0.005 * px28);
     double sb = sbase * fb * (1. + cb * gbm);
-    tot += wt * (got / (x * x + gx * gx) * sb);
+    tot += wt * (div_normal(got, x * x + gx * gx) * sb);
   }
   return 3. * c::BremConst * tot * hp.medium;
 }
@@ -114,11 +115,12 @@ __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_spectral_f64(
     SpectralKernelArgs A) {
   extern __shared__ double sm[];
-  __shared__ double s_in[64];  // the 21 inputs (+ 28 distribution values)
+  __shared__ double s_in[64];  // the 21 inputs (+ 28 distribution values), 1 / T
   __shared__ Spec s_D[4];
   __shared__ double s_Fe[2][6], s_fd0[2];
   __shared__ HrChan s_hc[3];
   __shared__ double s_x[kMaxQuad], s_w[kMaxQuad];
+  __shared__ double s_rx[kMaxQuad];  // 1 / x_i correctly rounded
 
   const int lane = threadIdx.x;
   const int n = A.n, N = 2 * n, E = A.ech;
@@ -140,9 +142,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   if (lane < n) {
     s_x[lane] = A.q1[Q1_REC * lane + Q1_X];
     s_w[lane] = A.q1[Q1_REC * lane + Q1_W];
+    s_rx[lane] = 1. / A.q1[Q1_REC * lane + Q1_X];
   }
   __syncthreads();
   const double T = vin[1], mu_e = vin[6];
+  // 1 / T: energies / T are formed as div_by_recip(nu, T, 1 / T), which
+  // rounds as nu / T does
+  if (lane == 63) s_in[kInvT] = 1. / T;
   const double eta = mu_e / T;
   const bool do_pair = A.flags.use_pair != 0;
   const bool do_brem = A.flags.use_brem != 0 && A.params.brem_impl != 2;
@@ -203,7 +209,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   const double tpair = c::PairScale * T;  // s_pair = 0.5 * 4.364 T (m1_opacities.hpp:1546)
   // partner node a of the pair/brem integral: t x_a or t / x_{a-n}
   // (integration.hpp:525-534), formed with the reference's rounding
-  auto pair_node = [&](int a) { return a < n ? mul_rn(tpair, s_x[a]) : tpair / s_x[a - n]; };
+  auto pair_node = [&](int a) {
+    return a < n ? mul_rn(tpair, s_x[a]) : div_by_recip(tpair, s_x[a - n], s_rx[a - n]);
+  };
+  auto over_T = [&](double e) { return div_by_recip(e, T, vin[kInvT]); };
   // ---- pair/brem node tables (once per point): F_1..F_5 at eta -+ z_a,
   //      TotalNuF at the node and the blocking factor --------------------------
   {
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     for (int t = lane; t < ntask; t += kWave) {
       if (t < nbund) {
         const int a = t < N ? t : t - N;
-        const double z = pair_node(a) / T;
+        const double z = over_T(pair_node(a));
         double F[6];
         fdi_bundle<1, 5>(t < N ? eta - z : eta + z, F);
         const int dst = (t < N ? L.fzm : L.fzp) + a;
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         const int a = t - nbund;
         const double nub = pair_node(a);
         V[L.nub + a] = nub;
-        V[L.z + a] = nub / T;
+        V[L.z + a] = over_T(nub);
         const double lw = log(nub);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -248,7 +257,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     // ---- Fermi-integral bundles F_1..F_5 at eta - y, eta + y, -eta - y ---------
     for (int t = lane; t < 3 * ne; t += kWave) {
       const int e = t / 3, kind = t - 3 * e;
-      const double y = energy(e) / T;
+      const double y = over_T(energy(e));
       double F[6];
       fdi_bundle<1, 5>(kind == 0 ? eta - y : kind == 1 ? eta + y : -eta - y, F);
       const int dst = (kind == 0 ? L.fym : kind == 1 ? L.fyp : L.fyq) + e;
@@ -265,7 +274,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
           if (kind == 0) {
             // TotalNuF(nu) of the four species (m1_opacities.hpp:1533-1536)
             V[L.nu + e] = nu;
-            V[L.y + e] = nu / T;
+            V[L.y + e] = over_T(nu);
             const double lw = log(nu);
   #pragma unroll
             for (int s = 0; s < 4; ++s) V[L.gnu + s * E + e] = nuf(nu, lw, Dget(s));
@@ -382,13 +391,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
               psi1 = x10 + y * (-x11 + x12 + x9 + y * (x13 + x8)) +
                      z * (x11 - x12 + x7 + y * (80. * (fzm(3) - fyp(3)) - x14 * y) +
                           z * (x13 + x5 + y * (x14 + y * (120. * (fyp(1) - fzm(1))))));
-              psi0 /= aux15;
-              psi1 /= aux15;
+              const double ia = div_normal(1., aux15);
+              psi0 *= ia;
+              psi1 *= ia;
             }
             // PairPhi: Phi_0 = (a1^2 psi0 + a2^2 psi1) C T^2 / (1 - e^{y+z}),
             // em = Phi_0 / 2, abs = e^{(w+w')/T} em
             const double aux = c::PairPhiC * (T * T) / (1. - safe_exp(y + z));
-            const double db = safe_exp((nu + nub) / T);
+            const double db = safe_exp(over_T(nu + nub));
             em[0] = 0.5 * ((a1e * psi0 + a2e * psi1) * aux);
             em[1] = 0.5 * ((a1e * psi1 + a2e * psi0) * aux);
             em[2] = 0.5 * ((a1x * psi0 + a2x * psi1) * aux);
@@ -400,12 +410,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
           if (do_brem) {
             if (A.params.brem_impl == 1) {
               // BremKernelsBRT06 at x = (w + w') / (2T) (kernel_brem_BRT06.hpp:42-69)
-              const double xb = 0.5 * (nu + nub) / T;
+              const double xb = over_T(0.5 * (nu + nub));
               const double k1x = bessk1_fit(xb) / xb;
               se = brt_pre * k1x * safe_exp(-xb);
               sa = brt_pre * k1x * safe_exp(xb);
             } else {
-              const double x = (nu + nub) / T;
+              const double x = over_T(nu + nub);
               sa = hr98_abs(x, hp, s_hc);
               se = sa * safe_exp(-x);
             }
@@ -413,7 +423,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
           // production: partner blocking; annihilation: partner occupation
           // (SpectralIntegrand, m1_opacities.hpp:1421-1496)
           const double nb2 = nub * nub;
-          const double jac = a < n ? 1. : 1. / (s_x[a - n] * s_x[a - n]);
+          const double jac = a < n ? 1. : s_rx[a - n] * s_rx[a - n];
   #pragma unroll
           for (int s = 0; s < 4; ++s) {
             const int ps = s ^ 1;
@@ -441,8 +451,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
           const double nu = V[L.nu + e], w = V[L.y + e];
           const double xq = s_x[a < n ? a : a - n];
           // partner energy nu x or nu / x (t = nu, m1_opacities.hpp:1547)
-          const double nub = a < n ? mul_rn(nu, xq) : nu / xq;
-          const double wp = nub / T;
+          const double nub = a < n ? mul_rn(nu, xq) : div_by_recip(nu, xq, s_rx[a - n]);
+          const double wp = over_T(nub);
           const double ad = fabs(w - wp);
           const double xm = w > wp ? w : wp, ym = w < wp ? w : wp;
           const double sgn = std::signbit(wp - w) ? 1. : -1.;
@@ -470,7 +480,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
               double fa0 = vFe[lep][3] - C3[3], fa1 = vFe[lep][4] - C3[4];
               double fa2 = vFe[lep][5] - C3[5];
               double ws = w + wp;
-              double inv = 1. / (w * w * wp * wp);
+              double inv = div_normal(1., w * w * wp * wp);
               double q4 = fw3 + 2. * (ws * fw2 + 3. * w * wp * fw1);
               // the out-kernel is linear in the couplings: b1 U + b2 V
               double t1 = (sgn * fa2 - fw4) * 0.2;
