@@ -315,18 +315,6 @@ BNS_HD void fdi_rat_pq(double s, double t, double& p, double& q) {
 #pragma unroll
   for (int i = 0; i < C::nqb; ++i) q = fma(s, q, C::qb[i]);
 }
-template <int K>
-BNS_HD double fdi_exp_branch(double ex, double t) {
-  double p, q;
-  fdi_exp_pq<K>(t, p, q);
-  return ex * (Fdi<K>::a0 - div_normal(ex * p, q));
-}
-template <int K>
-BNS_HD double fdi_rat_branch(double s, double t) {
-  double p, q;
-  fdi_rat_pq<K>(s, t, p, q);
-  return div_normal(p, q);
-}
 // exact polynomial added on reflection to y > 0 (functions.hpp:1606-2834)
 template <int K>
 BNS_HD double fdi_reflect(double fd, double y) {
@@ -351,16 +339,31 @@ BNS_HD double fdi_reflect(double fd, double y) {
   }
 }
 
+// One order.  Only the polynomials are evaluated under the branch; the
+// division and the combination F = A p/q + a0 e^x are shared (exp form: p
+// carries e^x and A = -e^x; rational form: A = 1, e^x = 0; NaN: A = e^x = 0
+// with finite p and q), so that a wavefront whose lanes take both forms pays
+// for one division per order.
 template <int K>
 struct FdiUnroll {
   template <int KHI>
-  BNS_HD static void eval(bool ebr, bool rbr, bool refl, double ex, double t, double s,
-                          double y, double* F) {
-    double fd = 0.;
-    if (ebr) fd = fdi_exp_branch<K>(ex, t);
-    else if (rbr) fd = fdi_rat_branch<K>(s, t);
+  BNS_HD static void eval(bool ebr, bool refl, double ex, double A, double t, double s, double y,
+                          double* F) {
+    double p, q;
+    if (ebr) {
+      fdi_exp_pq<K>(t, p, q);
+      p *= ex;
+    } else {
+      fdi_rat_pq<K>(s, t, p, q);
+    }
+    double fd = fma(A, div_normal(p, q), ex * Fdi<K>::a0);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // keeps this order's division here: sunk to the uses, the divisions of
+    // all orders would keep every (p, q) pair live at once
+    asm volatile("" : "+v"(fd));
+#endif
     F[K] = refl ? fdi_reflect<K>(fd, y) : fd;
-    if constexpr (K < KHI) FdiUnroll<K + 1>::template eval<KHI>(ebr, rbr, refl, ex, t, s, y, F);
+    if constexpr (K < KHI) FdiUnroll<K + 1>::template eval<KHI>(ebr, refl, ex, A, t, s, y, F);
   }
 };
 
@@ -375,16 +378,17 @@ BNS_HD void fdi_bundle(double y, double* F) {
   double x = -fabs(y);
   bool ebr = x < -2.;
   bool rbr = !ebr && (x <= 0.);  // NaN arguments give 0, as the reference
-  double ex = 0., t, s;
+  double ex = 0., A = rbr ? 1. : 0., t, s;
   if (ebr) {
     ex = exp(x);
+    A = -ex;
     t = ex * c::E2;
     s = 0.;
   } else {
-    s = -0.5 * x;
+    s = rbr ? -0.5 * x : 0.;
     t = 1. - s;
   }
-  FdiUnroll<KLO>::template eval<KHI>(ebr, rbr, y > 0., ex, t, s, y, F);
+  FdiUnroll<KLO>::template eval<KHI>(ebr, y > 0., ex, A, t, s, y, F);
 }
 
 template <int K>
