@@ -35,7 +35,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_y[64];      // node energies / T, formed as the reference does
   // wave-uniform per-point data, read back with broadcast LDS loads so that
   // it occupies neither scalar nor vector registers through the cell loops
-  __shared__ double s_in[64];     // the 21 inputs (+ 28 distribution values), 1 / T
+  __shared__ double s_in[64];     // the 21 inputs (+ 28 distribution values), 1 / T, log T
   __shared__ Spec s_D[4];
   __shared__ double s_Fe[2][6], s_fd0[2];
   __shared__ HrChan s_hc[3];
@@ -65,8 +65,10 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     d.lbf = MIX ? log(d.bf) : 0.;
     s_D[s] = d;
   }
-  // 1 / T: energies / T are formed as div_by_recip(nu, T, 1 / T)
+  // 1 / T: energies / T are formed as div_by_recip(nu, T, 1 / T); log(T):
+  // the logarithms of the NEPS energies are log(T) + log(nu / T)
   if (lane == 63) s_in[kInvT] = 1. / T;
+  if (lane == 62) s_in[kLogT] = log(T);
 
   // ---- per-point constants --------------------------------------------------
   const double eta = mu_e / T;
@@ -527,8 +529,9 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       // distribution values at nu and nubar, formed after the kernels so that
       // they are not live through the Fermi-integral evaluations
       double gn[4], gb[4], on[4], ob[4];  // f and 1 - f at nu and nubar
+      const double lT = vin[kLogT];
       if constexpr (MIX) {
-        float l1 = logf(static_cast<float>(nu)), l2 = logf(static_cast<float>(nub));
+        float l1 = static_cast<float>(lT + hr[NP_LW]), l2 = static_cast<float>(lT + hr[NP_LWP]);
         auto species = [&](int s) {
           Spec d = Dget(s);
           GOmg a = nuf_mixed(nu, l1, d), b = nuf_mixed(nub, l2, d);
@@ -541,7 +544,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
         for (int s = 0; s < 3; ++s) species(s);
         if (!same23) species(3);
       } else {
-        double l1 = log(nu), l2 = log(nub);
+        double l1 = lT + hr[NP_LW], l2 = lT + hr[NP_LWP];
         auto species = [&](int s) {
           Spec d = Dget(s);
           gn[s] = nuf(nu, l1, d);

@@ -125,6 +125,8 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
         r[NP_OMX] = 1.0 - xj;
         r[NP_OPX] = 1.0 + xj;
         r[NP_HALF] = half;
+        r[NP_LW] = std::log(w);
+        r[NP_LWP] = std::log(wp);
       }
     }
   }

@@ -54,7 +54,9 @@ enum {
   NP_OMX,     // 1 - x_j
   NP_OPX,     // 1 + x_j
   NP_HALF,    // 0: nu = t x_i (1 -+ x_j)/2, 1: nu = t (1 -+ x_j)/(2 x_i)
-  NP_REC = 12
+  NP_LW,      // log(w): log(nu) = log(T) + log(w)
+  NP_LWP,     // log(w')
+  NP_REC = 14
 };
 
 struct TableLayout {

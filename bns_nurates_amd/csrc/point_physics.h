@@ -17,9 +17,9 @@ namespace bns {
 namespace {
 
 constexpr int kWave = 64;
-// slot of 1 / T in a kernel's 64-double per-point LDS block (the inputs
-// occupy slots 0..48)
-constexpr int kInvT = 63;
+// slots of 1 / T and log(T) in a kernel's 64-double per-point LDS block
+// (the inputs occupy slots 0..48)
+constexpr int kInvT = 63, kLogT = 62;
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Give XCD x the contiguous range of points

@@ -44,7 +44,7 @@ namespace {
 //   cell block   8 integrand values of the 64 lanes of a round
 struct SpLds {
   int z, nub, fzm, fzp, gz, bz;
-  int nu, y, fym, fyp, fyq, gnu, bem, bab, iso, ip, in;
+  int nu, lnu, y, fym, fyp, fyq, gnu, bem, bab, iso, ip, in;
   int sf;
   int total;
 };
@@ -60,6 +60,7 @@ __host__ __device__ inline SpLds sp_lds(int n, int E) {
   L.gz = o;  o += 4 * N;
   L.bz = o;  o += 4 * N;
   L.nu = o;  o += E;
+  L.lnu = o; o += E;
   L.y = o;   o += E;
   L.fym = o; o += 5 * E;
   L.fyp = o; o += 5 * E;
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   __shared__ HrChan s_hc[3];
   __shared__ double s_x[kMaxQuad], s_w[kMaxQuad];
   __shared__ double s_rx[kMaxQuad];  // 1 / x_i correctly rounded
+  __shared__ double s_lx[kMaxQuad];  // log(x_i)
 
   const int lane = threadIdx.x;
   const int n = A.n, N = 2 * n, E = A.ech;
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     s_x[lane] = A.q1[Q1_REC * lane + Q1_X];
     s_w[lane] = A.q1[Q1_REC * lane + Q1_W];
     s_rx[lane] = 1. / A.q1[Q1_REC * lane + Q1_X];
+    s_lx[lane] = log(A.q1[Q1_REC * lane + Q1_X]);
   }
   __syncthreads();
   const double T = vin[1], mu_e = vin[6];
@@ -276,6 +279,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
             V[L.nu + e] = nu;
             V[L.y + e] = over_T(nu);
             const double lw = log(nu);
+            V[L.lnu + e] = lw;
   #pragma unroll
             for (int s = 0; s < 4; ++s) V[L.gnu + s * E + e] = nuf(nu, lw, Dget(s));
           } else if (kind == 1) {
@@ -528,9 +532,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
           // is not live through them: production weighted by the occupation,
           // annihilation by the blocking factor of the same species
           // (m1_opacities.hpp:1430-1433, 1481-1484)
-          const double lw = log(nub);
+          // log(nu x) or log(nu / x) from the tabled logarithms
+          const double lw = V[L.lnu + e] + (a < n ? s_lx[a] : -s_lx[a - n]);
           const double nb2 = nub * nub;
-          const double jac = a < n ? 1. : 1. / (xq * xq);
+          const double jac = a < n ? 1. : s_rx[a - n] * s_rx[a - n];
           double gb[4];
 #pragma unroll
           for (int s = 0; s < 3; ++s) gb[s] = nuf(nub, lw, Dget(s));
