@@ -121,10 +121,11 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     double a4[4] = {0., 0., 0., 0.};
     for (int it = lane; it < 8 * n1; it += kWave) {
       int s = it / (2 * n1), r = it - s * 2 * n1, half = r >= n1, i = r - half * n1;
-      double x = tb[L.q1 + Q1_REC * i + Q1_X], w = tb[L.q1 + Q1_REC * i + Q1_W];
+      const double* q = tb + L.q1 + Q1_REC * i;
+      double x = q[Q1_X], w = q[Q1_W], rx = q[Q1_RX];
       double ts = s == 0 ? s_iso[0] : s == 1 ? s_iso[1] : s == 2 ? s_iso[2] : s_iso[3];
-      double nu = half ? ts / x : ts * x;
-      double W = half ? w / (x * x) : w;
+      double nu = half ? div_by_recip(ts, x, rx) : ts * x;
+      double W = half ? w * (rx * rx) : w;
       double g = dist<MIX>(nu, Dget(s)).g;
       double isc = etaNNp * iso_leg(nu, 1, wm) + etaNNn * iso_leg(nu, 2, wm);
       double nu2 = nu * nu;
@@ -159,10 +160,11 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     double a8[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
     for (int it = lane; it < 4 * n1; it += kWave) {
       int s = it / (2 * n1), r = it - s * 2 * n1, half = r >= n1, i = r - half * n1;
-      double x = tb[L.q1 + Q1_REC * i + Q1_X], w = tb[L.q1 + Q1_REC * i + Q1_W];
+      const double* q = tb + L.q1 + Q1_REC * i;
+      double x = q[Q1_X], w = q[Q1_W], rx = q[Q1_RX];
       double ts = s == 0 ? s_beta[0] : s_beta[1];
-      double nu = half ? ts / x : ts * x;
-      double W = half ? w / (x * x) : w;
+      double nu = half ? div_by_recip(ts, x, rx) : ts * x;
+      double W = half ? w * (rx * rx) : w;
       double g = dist<MIX>(nu, Dget(s)).g;
       double em, ab;
       beta_em_ab(nu, s, bp, em, ab);

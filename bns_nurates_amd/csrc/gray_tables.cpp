@@ -33,6 +33,7 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
   for (int i = 0; i < n1; ++i) {
     block[L.q1 + Q1_REC * i + Q1_X] = x1[i];
     block[L.q1 + Q1_REC * i + Q1_W] = w1[i];
+    block[L.q1 + Q1_REC * i + Q1_RX] = 1.0 / x1[i];
   }
   const int n = n2, N = L.N;
   // 2D nodes a < n: x_a, a >= n: 1/x_{a-n} (m1_opacities.hpp:783-787); the

@@ -18,8 +18,8 @@ namespace bns {
 
 constexpr int kMaxQuad = 32;  // quadrature order n <= 32 (2n = 64 nodes)
 
-// 1D quadrature record
-enum { Q1_X = 0, Q1_W, Q1_REC = 2 };
+// 1D quadrature record: node, weight, 1 / node correctly rounded
+enum { Q1_X = 0, Q1_W, Q1_RX, Q1_REC = 3 };
 // 2D node record: 1 / x correctly rounded, number weight times y^2 (y =
 // nu / T), the quadrature node x and whether the energy is t x (0) or t / x (1)
 enum { ND_RX = 0, ND_WQ, ND_X, ND_INV, ND_REC = 4 };

@@ -113,18 +113,19 @@ __device__ __forceinline__ Spec pick(const Spec* D, int s) {
 __device__ __forceinline__ void form_factors(double E, int reac, double& cv, double& ca,
                                              double& F2) {
   const double lamp = 1.793, lamn = -1.913;
+  // every denominator below is >= 1
   double e = E * c::WMEScale;
-  double tau = 0.5 * (e * e) / (1. + e);
-  double eta = 1. / fma(5.6, tau, 1.);
+  double tau = div_normal(0.5 * (e * e), 1. + e);
+  double eta = div_normal(1., fma(5.6, tau, 1.));
   double g = fma(4.97, tau, 1.);
-  double G = 1. / (g * g);
-  double itau = 1. / (1. + tau);
+  double G = div_normal(1., g * g);
+  double itau = div_normal(1., 1. + tau);
   double Fp1 = fma(tau, 1. + lamp, 1.) * G * itau;
   double Fp2 = lamp * G * itau;
   double Fn1 = tau * lamn * (1. - eta) * G * itau;
   double Fn2 = lamn * fma(tau, eta, 1.) * G * itau;
   double c3 = fma(3.53, tau, 1.);
-  double ic3 = 1. / (c3 * c3);
+  double ic3 = div_normal(1., c3 * c3);
   const double hw = 0.5 - 2. * c::SinThW2;
   if (reac == 1) {
     cv = hw * Fp1 - 0.5 * Fn1;
@@ -153,9 +154,10 @@ __device__ __forceinline__ double iso_leg(double omega, int reac, bool wm) {
   double h0 = hv * hv + 3. * (ha * ha), h1 = hv * hv - ha * ha;
   double e = omega * c::WMEScale;
   double eF = e * F2;
-  double R0 = (cv * cv + 3. * (ca * ca) + 1.5 * (eF * eF) + 2. * (e * e) * cv * F2) / h0;
-  double R1 = (cv * cv - ca * ca - 2. * (eF * eF) - 4. * (e * e) * cv * F2) / h1;
-  return c0 * R0 - c1 * R1 / 3.;
+  // h0, h1 and 3 are constants: their reciprocals fold at compile time
+  double R0 = (cv * cv + 3. * (ca * ca) + 1.5 * (eF * eF) + 2. * (e * e) * cv * F2) * (1. / h0);
+  double R1 = (cv * cv - ca * ca - 2. * (eF * eF) - 4. * (e * e) * cv * F2) * (1. / h1);
+  return c0 * R0 - c1 * R1 * (1. / 3.);
 }
 
 // R0 of WMScatt (weak_magnetism.hpp:165-210): the correction of the zeroth
@@ -167,7 +169,7 @@ __device__ __forceinline__ double wm_scatt_r0(double omega, int reac) {
   double h0 = hv * hv + 3. * (ha * ha);
   double e = omega * c::WMEScale;
   double eF = e * F2;
-  return (cv * cv + 3. * (ca * ca) + 1.5 * (eF * eF) + 2. * (e * e) * cv * F2) / h0;
+  return (cv * cv + 3. * (ca * ca) + 1.5 * (eF * eF) + 2. * (e * e) * cv * F2) * (1. / h0);
 }
 
 // WMAbsEm (weak_magnetism.hpp:123-160): R (neutrino) or Rbar (antineutrino)
@@ -181,7 +183,7 @@ __device__ __forceinline__ double wm_abs(double omega, int s) {
   double tmp2 = 4. * (cv + F2) * ca * e * a;
   double b = fma(2., e, 1.);
   double tmp3 = (c::Gv * c::Gv + 3. * (c::Ga * c::Ga)) * (b * b * b);
-  return (s == 0 ? tmp1 + tmp2 : tmp1 - tmp2) / tmp3;
+  return div_normal(s == 0 ? tmp1 + tmp2 : tmp1 - tmp2, tmp3);
 }
 
 // EtaNNAbs (formulas_beta_iso.hpp:64-97)
@@ -198,7 +200,7 @@ struct BetaPoint {
   double etanp, etapn;  // EtaNNAbs for n -> p and p -> n
   double Qp;            // dQ + dU
   double munu;          // mu_p + mu_e - mu_n
-  double mu_e, T;
+  double mu_e, T, rT;   // rT = 1 / T
   bool wm, decay;       // use_WM_ab, use_decay
 };
 
@@ -218,6 +220,7 @@ __device__ __forceinline__ BetaPoint beta_point(double nb, double T, double yp, 
   bp.munu = mu_p + mu_e - mu_n;
   bp.mu_e = mu_e;
   bp.T = T;
+  bp.rT = 1. / T;
   bp.wm = wm;
   bp.decay = decay;
   return bp;
@@ -235,13 +238,13 @@ __device__ __forceinline__ void beta_em_ab(double nu, int s, const BetaPoint& bp
   double sg = s == 0 ? 1. : -1.;
   double Ec = nu + sg * bp.Qp, Ed = -Ec;
   double muc = sg * bp.mu_e;
-  double fc = fermi_of_arg((Ec - muc) / T);
-  double fdd = fermi_of_arg((Ed + muc) / T);
+  double fc = fermi_of_arg(div_by_recip(Ec - muc, T, bp.rT));
+  double fdd = fermi_of_arg(div_by_recip(Ed + muc, T, bp.rT));
   double cap = 0., dec = 0.;
-  if (Ec - m >= 0.) cap = Ec * Ec * sqrt(1. - m2 / (Ec * Ec)) * R;
-  if (bp.decay && (Ed - m >= 0.)) dec = Ed * Ed * sqrt(1. - m2 / (Ed * Ed));
+  if (Ec - m >= 0.) cap = Ec * Ec * sqrt(1. - div_normal(m2, Ec * Ec)) * R;
+  if (bp.decay && (Ed - m >= 0.)) dec = Ed * Ed * sqrt(1. - div_normal(m2, Ed * Ed));
   em = c::BetaConst * (s == 0 ? bp.etapn : bp.etanp) * (cap * fc + dec * (1. - fdd));
-  ab = em * safe_exp((nu - sg * bp.munu) / T);
+  ab = em * safe_exp(div_by_recip(nu - sg * bp.munu, T, bp.rT));
 }
 
 // EtaNNSc (formulas_beta_iso.hpp:371-395)

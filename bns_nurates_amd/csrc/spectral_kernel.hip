@@ -144,7 +144,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   if (lane < n) {
     s_x[lane] = A.q1[Q1_REC * lane + Q1_X];
     s_w[lane] = A.q1[Q1_REC * lane + Q1_W];
-    s_rx[lane] = 1. / A.q1[Q1_REC * lane + Q1_X];
+    s_rx[lane] = A.q1[Q1_REC * lane + Q1_RX];
     s_lx[lane] = log(A.q1[Q1_REC * lane + Q1_X]);
   }
   __syncthreads();
