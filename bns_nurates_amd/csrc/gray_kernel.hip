@@ -25,8 +25,10 @@ namespace bns {
 // ---------------------------------------------------------------------------
 // The body of both kernel variants: MIX = false is the FP64 path, MIX = true
 // the mixed-precision path (FP32 distribution values and HR98 channel
-// functions, everything else FP64).
-template <bool MIX>
+// functions, everything else FP64).  STIM (stimulated absorption,
+// A.stim == 1) and BLOCK (partner blocking, neglect_blocking == 0) are
+// fixed per launch, so that a cell evaluates only its own weighting.
+template <bool MIX, bool STIM, bool BLOCK>
 __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
   __shared__ double s_Fp[5][64];  // F_1..F_5 (eta + y_a)
@@ -168,7 +170,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       double g = dist<MIX>(nu, Dget(s)).g;
       double em, ab;
       beta_em_ab(nu, s, bp, em, ab);
-      if (A.stim == 1) ab = ab + em;
+      if constexpr (STIM) ab = ab + em;
       double nu2 = nu * nu;
       double ve = W * nu2 * em, va = W * nu2 * g * ab;
       a8[0] += s == 0 ? ve : 0.;
@@ -189,8 +191,8 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     }
   }
 
-  const bool blocking = A.params.neglect_blocking == 0;
-  const bool stim = A.stim == 1;
+  constexpr bool blocking = BLOCK;
+  constexpr bool stim = STIM;
 
   // ========================= 2D: pair + bremsstrahlung =======================
   const bool do_pair = A.flags.use_pair == 1;
@@ -424,7 +426,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           double gu = s_g[s][u];
           double eb = ke * s_bf[ps][v];
           double e_ent, a_ent;
-          if (stim) {
+          if constexpr (stim) {
             e_ent = eb;
             a_ent = gu * fma(ka, s_g[ps][v], eb);
           } else {
@@ -576,7 +578,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
         double te1 = em1 * gb[s], ta1 = ab1 * bfb;
         double te2 = ab1 * gn[s], ta2 = em1 * bfn;
         double Ae, Aa, Be, Ba;
-        if (stim) {
+        if constexpr (stim) {
           Ae = te1;
           Aa = gn[s] * (te1 + ta1);
           Be = te2;
@@ -649,18 +651,18 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   if (lane < 20) A.out[lane * ld + p] = vin[lane];
 }
 
-template <int WAVES_PER_EU>
+template <int WAVES_PER_EU, bool STIM, bool BLOCK>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_f64(
     GrayKernelArgs A) {
-  gray_point<false>(A);
+  gray_point<false, STIM, BLOCK>(A);
 }
 
-template <int WAVES_PER_EU>
+template <int WAVES_PER_EU, bool STIM, bool BLOCK>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_mixed(
     GrayKernelArgs A) {
-  gray_point<true>(A);
+  gray_point<true, STIM, BLOCK>(A);
 }
 
 }  // namespace bns
@@ -669,19 +671,36 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
 #define BNS_GRAY_WAVES_PER_EU 3
 #endif
 namespace bns {
-hipError_t launch_gray_m1_f64(const GrayKernelArgs& a, hipStream_t stream) {
+namespace {
+template <bool MIX, bool STIM, bool BLOCK>
+void launch_instance(const GrayKernelArgs& a, dim3 grid, hipStream_t stream) {
+  constexpr int W = BNS_GRAY_WAVES_PER_EU;
+  if constexpr (MIX)
+    hipLaunchKernelGGL((k_gray_m1_mixed<W, STIM, BLOCK>), grid, dim3(64), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_gray_m1_f64<W, STIM, BLOCK>), grid, dim3(64), 0, stream, a);
+}
+// the instance for the launch's stimulated-absorption and blocking switches
+template <bool MIX>
+hipError_t launch_gray(const GrayKernelArgs& a, hipStream_t stream) {
   if (a.npts <= 0) return hipSuccess;
-  dim3 grid(static_cast<unsigned>(a.npts)), block(64);
-  hipLaunchKernelGGL(k_gray_m1_f64<BNS_GRAY_WAVES_PER_EU>, grid, block, 0, stream, a);
+  const dim3 grid(static_cast<unsigned>(a.npts));
+  const bool blocking = a.params.neglect_blocking == 0;
+  if (a.stim == 1) {
+    if (blocking) launch_instance<MIX, true, true>(a, grid, stream);
+    else launch_instance<MIX, true, false>(a, grid, stream);
+  } else {
+    if (blocking) launch_instance<MIX, false, true>(a, grid, stream);
+    else launch_instance<MIX, false, false>(a, grid, stream);
+  }
   return hipGetLastError();
 }
-}  // namespace bns
+}  // namespace
 
-namespace bns {
+hipError_t launch_gray_m1_f64(const GrayKernelArgs& a, hipStream_t stream) {
+  return launch_gray<false>(a, stream);
+}
 hipError_t launch_gray_m1_mixed(const GrayKernelArgs& a, hipStream_t stream) {
-  if (a.npts <= 0) return hipSuccess;
-  dim3 grid(static_cast<unsigned>(a.npts)), block(64);
-  hipLaunchKernelGGL(k_gray_m1_mixed<BNS_GRAY_WAVES_PER_EU>, grid, block, 0, stream, a);
-  return hipGetLastError();
+  return launch_gray<true>(a, stream);
 }
 }  // namespace bns
