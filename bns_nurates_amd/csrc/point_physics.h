@@ -48,7 +48,9 @@ struct Spec {
 
 __device__ __forceinline__ double nuf(double w, double lw, const Spec& d) {
   double arg = (w - d.mut) * d.invTt;
-  double thin = d.bf * exp(fma(d.cf, lw, -w * d.invTf));
+  // the exponent is below 0.6 log(w) + log(bf) (c_f = 0.6 from M1; a caller's
+  // c_f beyond 709 / log(w) gives inf, as exp does); below -746 e^x is 0
+  double thin = d.bf * exp_bounded(fmax(fma(d.cf, lw, -w * d.invTf), -746.));
   return fma(d.wt, fermi_of_arg(arg), d.wf * thin);
 }
 
