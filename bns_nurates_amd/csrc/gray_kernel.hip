@@ -483,9 +483,19 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           // MezzacappaIntOut (kernel_nes.hpp:50-78) with the differences of
           // ComputeFDIForInelastic (:29-44)
           double A5[6], B5[6], C3[6];
-          fdi_bundle<1, 5>(el - wp, A5);
-          fdi_bundle<1, 5>(el - w, B5);
-          fdi_bundle<3, 5>(el - ad, C3);
+          // positron targets (el = -eta): at eta > 40 every argument is in
+          // the deep tail of the exponential form, so the whole wavefront
+          // skips the rationals (the same bits); elsewhere the wave-uniform
+          // test costs more than it saves
+          if (lep == 1) {
+            fdi_bundle<1, 5, true>(el - wp, A5);
+            fdi_bundle<1, 5, true>(el - w, B5);
+            fdi_bundle<3, 5, true>(el - ad, C3);
+          } else {
+            fdi_bundle<1, 5>(el - wp, A5);
+            fdi_bundle<1, 5>(el - w, B5);
+            fdi_bundle<3, 5>(el - ad, C3);
+          }
           double fw0 = A5[1] - B5[1], fw1 = A5[2] - B5[2], fw2 = A5[3] - B5[3];
           double fw3 = A5[4] - B5[4], fw4 = A5[5] - B5[5];
           double fa0 = vFe[lep][3] - C3[3], fa1 = vFe[lep][4] - C3[4];

@@ -397,15 +397,41 @@ struct FdiUnroll {
   }
 };
 
+// Deep tail of the exponential form: for x < kFdiDeep the correction
+// e^{2x} p/q is below 2^-54 e^x a0 for every order (|p/q| <= 1.9 as t -> 0,
+// a0 >= 1, so the bound holds from x = -36.7 on), and the fused
+// F = fma(-e^x, e^x p/q, e^x a0) rounds to e^x a0: the same bits without the
+// rational.
+constexpr double kFdiDeep = -40.;
+template <int K>
+struct FdiDeepUnroll {
+  template <int KHI>
+  BNS_HD static void eval(bool refl, double ex, double y, double* F) {
+    const double fd = ex * Fdi<K>::a0;
+    F[K] = refl ? fdi_reflect<K>(fd, y) : fd;
+    if constexpr (K < KHI) FdiDeepUnroll<K + 1>::template eval<KHI>(refl, ex, y, F);
+  }
+};
+
 // F_k(y) for k = KLO..KHI at one argument, sharing the branch decision and
 // the exponential (F[k] receives F_k; F must hold KHI + 1 entries).
 // Inlined: with MachineLICM disabled at build time the coefficient
 // materializations stay next to their use inside the cell loops.  (A
 // batched reciprocal of the five denominators saves divisions but keeps
 // more values live; at three waves per SIMD the extra spills cost more.)
-template <int KLO, int KHI = 5>
+// With DEEP, a wavefront whose active lanes all lie in the deep tail skips
+// the rationals and the divisions (a wave-uniform branch, same results);
+// callers enable it where whole wavefronts commonly fall in the tail, since
+// the test itself costs a few instructions per bundle.
+template <int KLO, int KHI = 5, bool DEEP = false>
 BNS_HD void fdi_bundle(double y, double* F) {
   double x = -fabs(y);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (DEEP && __all(x < kFdiDeep)) {
+    FdiDeepUnroll<KLO>::template eval<KHI>(y > 0., exp_bounded(fmax(x, -746.)), y, F);
+    return;
+  }
+#endif
   bool ebr = x < -2.;
   bool rbr = !ebr && (x <= 0.);  // NaN arguments give 0, as the reference
   double ex = 0., A = rbr ? 1. : 0., t, s;
