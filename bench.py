@@ -261,7 +261,7 @@ def surrogate_eval(ctx, dpts, ref, fp64_ms, params, nquad, train_points, epochs,
     npts = dpts.shape[1]
     r = ref.cpu().numpy()
     rows = S.OUT_ROWS
-    res = {"members": 4, "network": "12 -> 4 x 256 SiLU -> 15, BF16 MFMA, FP32 accumulation",
+    res = {"members": 4, "network": "15 -> 4 x 256 SiLU -> 15, BF16 MFMA, FP32 accumulation",
            "train_points": int(ens.n_train), "epochs": epochs, "train_wall_s": round(train_s, 1),
            "final_epoch_loss": [float(f"{h[-1]:.4g}") for h in ens.history],
            "fp64_kernel_ms": fp64_ms, "thresholds": []}

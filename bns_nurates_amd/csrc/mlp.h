@@ -1,5 +1,5 @@
 // mlp.h -- shapes and parameter layout of the surrogate MLP (one ensemble
-// member): 12 features (padded to 32) -> 4 x 256 (SiLU) -> 15 outputs
+// member): 15 features (padded to 32) -> 4 x 256 (SiLU) -> 15 outputs
 // (padded to 16).  Parameters of a member are one FP32 blob:
 // W_0, b_0, W_1, b_1, ..., W_l row-major [K_l][N_l].
 #ifndef BNS_NURATES_AMD_MLP_H_
@@ -8,7 +8,7 @@
 namespace bns {
 namespace mlp {
 
-constexpr int kFeat = 12;     // input features
+constexpr int kFeat = 15;     // input features
 constexpr int kOut = 15;      // outputs
 constexpr int kK0 = 32;       // padded input width (one MFMA K step)
 constexpr int kNL = 16;       // padded output width

@@ -4,7 +4,7 @@
 // inference kernel that evaluates every ensemble member with activations in
 // LDS.  No counterpart in the reference.
 //
-// Network of one member: 12 input features (padded to K = 32) -> 256 -> 256
+// Network of one member: 15 input features (padded to K = 32) -> 256 -> 256
 // -> 256 -> 256 -> 15 outputs (padded to N = 16), SiLU on the hidden layers.
 // Weights W_l [K_l][N_l] and biases b_l [N_l] are FP32 master copies; the
 // matrix products take BF16 weights and activations and accumulate in FP32
@@ -34,11 +34,16 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
+#if defined(__HIP_DEVICE_COMPILE__)
+  // v_cvt_pk_bf16_f32: the same rounding (NaN stays NaN)
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+#else
   uint32_t u;
   std::memcpy(&u, &f, 4);
   if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
   u += 0x7fffu + ((u >> 16) & 1u);
   return uint16_t(u >> 16);
+#endif
 }
 __host__ __device__ inline float bf2f(uint16_t h) {
   uint32_t u = uint32_t(h) << 16;
@@ -52,9 +57,12 @@ __device__ inline bf16x8 ld8(const uint16_t* p) {  // 16-byte aligned LDS or glo
   return __builtin_bit_cast(bf16x8, v);
 }
 
-__device__ inline float silu(float z) { return z / (1.f + __expf(-z)); }
+// SiLU z / (1 + e^-z) with the hardware exponential and reciprocal (both
+// within a few ulp of FP32; the activations are rounded to BF16 next)
+__device__ inline float sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid_fast(z); }
 __device__ inline float silu_grad(float z) {
-  float s = 1.f / (1.f + __expf(-z));
+  float s = sigmoid_fast(z);
   return s * (1.f + z * (1.f - s));
 }
 
@@ -199,7 +207,7 @@ __global__ void k_adam(int64_t n, float* __restrict__ w, const float* __restrict
 }
 
 // batch assembly: rows perm[start + r] of the normalised features (FP32,
-// [n][12]) and targets ([n][15]) into the padded BF16 input [B][32] and FP32
+// [n][15]) and targets ([n][15]) into the padded BF16 input [B][32] and FP32
 // targets [B][16]
 __global__ void k_gather_batch(int B, const int32_t* __restrict__ perm, int64_t start,
                                const float* __restrict__ X, const float* __restrict__ Y,
@@ -243,36 +251,66 @@ struct InferArgs {
 };
 
 constexpr int kRows = 64, kLds = kHidden + 8;  // points per workgroup, padded row
+constexpr int kXLd = kK0 + 8;                    // padded row of the input features
 
-// A (from LDS, [64][stride]) times the packed weights of one layer, K steps
-// of 32, for this wave's column tiles; result in acc[rt][ct]
-template <int NCT>
-__device__ inline void mma_layer(const uint16_t* __restrict__ A, int stride, int K,
-                                 const uint16_t* __restrict__ Wp, int NB, int nb0, int lane,
-                                 floatx4 (&acc)[4][4]) {
+
+// One layer for this wave's 64 output features (16 nb0 .. 16 nb0 + 63) and
+// the workgroup's 64 points, with the operands of the MFMA swapped: the
+// packed weight fragment is the A operand (rows = output features) and the
+// activation fragment (rows = points in LDS) the B operand, so the result is
+// the transposed tile and lane l holds output features 4 (l / 16) + v,
+// v = 0..3, of point l % 16 -- four consecutive features of one point,
+// stored with one 8-byte LDS write.  K is fixed so that the k loop unrolls;
+// the weights of step kb + 1 are loaded while step kb multiplies.
+template <int K>
+__device__ inline void mma_layer_t(const uint16_t* __restrict__ A, int stride,
+                                   const uint16_t* __restrict__ Wp, int NB, int nb0, int lane,
+                                   floatx4 (&acc)[4][4]) {
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int kb = 0; kb < K / 32; ++kb) {
-    bf16x8 b[NCT];
+    for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+  bf16x8 w[2][4];
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct)
-      b[ct] = ld8(Wp + ((int64_t(kb) * NB + nb0 + ct) * 64 + lane) * 8);
+  for (int ct = 0; ct < 4; ++ct) w[0][ct] = ld8(Wp + ((nb0 + ct) * 64 + lane) * 8);
+#pragma unroll
+  for (int kb = 0; kb < K / 32; ++kb) {
+    const int cur = kb & 1;
+    if (kb + 1 < K / 32) {
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+        w[cur ^ 1][ct] = ld8(Wp + ((int64_t(kb + 1) * NB + nb0 + ct) * 64 + lane) * 8);
+    }
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) {
-      bf16x8 a = ld8(A + (rt * 16 + (lane & 15)) * stride + 32 * kb + 8 * (lane >> 4));
+      bf16x8 x = ld8(A + (rt * 16 + (lane & 15)) * stride + 32 * kb + 8 * (lane >> 4));
 #pragma unroll
-      for (int ct = 0; ct < NCT; ++ct)
-        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ct], acc[rt][ct], 0, 0, 0);
+      for (int ct = 0; ct < 4; ++ct)
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[cur][ct], x, acc[rt][ct], 0, 0, 0);
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_infer(InferArgs a) {
-  __shared__ __attribute__((aligned(16))) uint16_t X[kRows][kK0 + 8];
-  __shared__ __attribute__((aligned(16))) uint16_t Ha[kRows][kLds];
-  __shared__ __attribute__((aligned(16))) uint16_t Hb[kRows][kLds];
+// bias + SiLU of one transposed tile, four features of one point per lane,
+// written as one 8-byte row segment of H
+__device__ inline void store_act4(uint16_t* __restrict__ dst, floatx4 v, const float4 b) {
+  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  bf16x4 h;
+  h[0] = static_cast<__bf16>(silu(v[0] + b.x));
+  h[1] = static_cast<__bf16>(silu(v[1] + b.y));
+  h[2] = static_cast<__bf16>(silu(v[2] + b.z));
+  h[3] = static_cast<__bf16>(silu(v[3] + b.w));
+  *reinterpret_cast<uint2*>(dst) = __builtin_bit_cast(uint2, h);
+}
+
+// One workgroup (4 waves) evaluates every ensemble member on 64 points.
+// LDS holds the points' BF16 features and one activation buffer, updated in
+// place: a layer's outputs stay in the MFMA accumulators until every wave
+// has read its inputs.  212 VGPRs: 2 waves per SIMD (3 spill, and measured
+// no faster).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_infer(InferArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t X[kRows][kXLd];
+  __shared__ __attribute__((aligned(16))) uint16_t H[kRows][kLds];
   __shared__ float spread[kRows][kNL];
   __shared__ float xs[kRows][kFeat];  // standardised features (linear output model)
   __shared__ int oob[kRows];
@@ -298,6 +336,10 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a) {
         f[4 + 3 * s] = float(log(q[(13 + s) * ld]));
         f[5 + 3 * s] = float(q[(17 + s) * ld]);
       }
+      const double rT = 1. / q[1 * ld];
+      f[12] = float(q[6 * ld] * rT);                // mu_e / T
+      f[13] = float((q[5 * ld] - q[4 * ld]) * rT);  // (mu_n - mu_p) / T
+      f[14] = float(rT);
     } else {
 #pragma unroll
       for (int i = 0; i < kFeat; ++i) f[i] = nm.feat_mean[i];
@@ -311,71 +353,70 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a) {
       X[t][i] = f2bf(x);
     }
 #pragma unroll
-    for (int i = kFeat; i < kK0 + 8; ++i) X[t][i] = 0;
+    for (int i = kFeat; i < kXLd; ++i) X[t][i] = 0;
     oob[t] = out_of_box;
   }
   __syncthreads();
 
-  // running ensemble sums of this lane's four outputs (rows 16 wave +
-  // 4 (lane / 16) + v, column lane % 16)
+  // running ensemble sums of this lane's four outputs: outputs
+  // 4 (lane / 16) + v of point 16 wave + lane % 16
   float sum[4] = {0.f, 0.f, 0.f, 0.f}, sq[4] = {0.f, 0.f, 0.f, 0.f};
+  const int q4 = 4 * (lane >> 4);
   for (int e = 0; e < a.members; ++e) {
     const uint16_t* wp = a.wp[e];
     const float* wf = a.w[e];
-    const uint16_t* in = &X[0][0];
-    int in_stride = kK0 + 8;
+#pragma unroll 1
     for (int l = 0; l < kLayers - 1; ++l) {
-      const int K = l ? kHidden : kK0;
       floatx4 acc[4][4];
-      mma_layer<4>(in, in_stride, K, wp + kPOff[l], kHidden / 16, 4 * wave, lane, acc);
-      uint16_t* dst = (l & 1) ? &Hb[0][0] : &Ha[0][0];
+      if (l == 0)
+        mma_layer_t<kK0>(&X[0][0], kXLd, wp + kPOff[0], kHidden / 16, 4 * wave, lane, acc);
+      else
+        mma_layer_t<kHidden>(&H[0][0], kLds, wp + kPOff[l], kHidden / 16, 4 * wave, lane, acc);
+      __syncthreads();  // every wave has read H: overwrite it in place
       const float* bias = wf + kBOff[l];
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
-        const int col = 64 * wave + 16 * ct + (lane & 15);
-        const float bc = bias[col];
+        const int col = 64 * wave + 16 * ct + q4;
+        const float4 bc = *reinterpret_cast<const float4*>(bias + col);
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            dst[(rt * 16 + 4 * (lane >> 4) + v) * kLds + col] = f2bf(silu(acc[rt][ct][v] + bc));
+        for (int rt = 0; rt < 4; ++rt) store_act4(&H[rt * 16 + (lane & 15)][col], acc[rt][ct], bc);
       }
       __syncthreads();
-      in = dst;
-      in_stride = kLds;
     }
-    // output layer: wave w takes rows 16 w .. 16 w + 15
+    // output layer (16 padded outputs): wave w takes points 16 w .. 16 w + 15
     floatx4 o = floatx4{0.f, 0.f, 0.f, 0.f};
     const uint16_t* wl = wp + kPOff[kLayers - 1];
+#pragma unroll
     for (int kb = 0; kb < kHidden / 32; ++kb) {
       bf16x8 b = ld8(wl + (int64_t(kb) * 64 + lane) * 8);
-      bf16x8 x = ld8(in + (16 * wave + (lane & 15)) * kLds + 32 * kb + 8 * (lane >> 4));
-      o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, b, o, 0, 0, 0);
+      bf16x8 x = ld8(&H[16 * wave + (lane & 15)][32 * kb + 8 * (lane >> 4)]);
+      o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, x, o, 0, 0, 0);
     }
-    const float bo = wf[kBOff[kLayers - 1] + (lane & 15)];
+    const float4 bo = *reinterpret_cast<const float4*>(wf + kBOff[kLayers - 1] + q4);
+    const float bov[4] = {bo.x, bo.y, bo.z, bo.w};
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      float y = o[v] + bo;
+      float y = o[v] + bov[v];
       sum[v] += y;
       sq[v] += y * y;
     }
-    __syncthreads();  // the next member overwrites Ha / Hb
+    __syncthreads();  // the next member overwrites H
   }
 
   // ensemble mean -> outputs; spread (natural-log units) -> fallback flag
-  const int col = lane & 15;
+  const int r = 16 * wave + (lane & 15);
   const float inv = 1.f / float(a.members);
-  const float os = col < kOut ? nm.out_std[col] : 0.f, om = col < kOut ? nm.out_mean[col] : 0.f;
-  const float rlo = col < kOut ? nm.res_lo[col] : 0.f, rhi = col < kOut ? nm.res_hi[col] : 0.f;
   float mean[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
+    const int c = q4 + v;
+    const float os = c < kOut ? nm.out_std[c] : 0.f;
+    const float rlo = c < kOut ? nm.res_lo[c] : 0.f, rhi = c < kOut ? nm.res_hi[c] : 0.f;
     mean[v] = sum[v] * inv;
     float var = fmaxf(sq[v] * inv - mean[v] * mean[v], 0.f);
     // a residual outside the training range counts as unbounded spread
     bool in_range = mean[v] >= rlo && mean[v] <= rhi;
-    spread[16 * wave + 4 * (lane >> 4) + v][col] =
-        col < kOut ? (in_range ? sqrtf(var) * os : __builtin_huge_valf()) : 0.f;
+    spread[r][c] = c < kOut ? (in_range ? sqrtf(var) * os : __builtin_huge_valf()) : 0.f;
   }
   __syncthreads();
   if (t < kRows) {
@@ -393,17 +434,17 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a) {
   __syncthreads();
   // outputs of the accepted points: column c = 5 s + q (q: eta_0, eta,
   // kappa_0_a, kappa_a, kappa_s) goes to field 4 q' + s, anux copying nux
+  const int64_t p = p0 + r;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int r = 16 * wave + 4 * (lane >> 4) + v;
-    const int64_t p = p0 + r;
-    if (col < kOut && p < a.npts && !oob[r]) {
-      const int s = col / 5, q = col % 5;
+    const int c = q4 + v;
+    if (c < kOut && p < a.npts && !oob[r]) {
+      const int s = c / 5, q = c % 5;
       const int base = q == 0 ? 0 : q == 1 ? 8 : q == 2 ? 4 : q == 3 ? 12 : 16;
-      float lin = nm.lin[0][col];
+      float lin = nm.lin[0][c];
 #pragma unroll
-      for (int i = 0; i < kFeat; ++i) lin = fmaf(nm.lin[1 + i][col], xs[r][i], lin);
-      const double y = exp(double(lin) + double(mean[v] * os + om));
+      for (int i = 0; i < kFeat; ++i) lin = fmaf(nm.lin[1 + i][c], xs[r][i], lin);
+      const double y = exp(double(lin) + double(mean[v] * nm.out_std[c] + nm.out_mean[c]));
       a.out[(base + s) * a.ld + p] = y;
       if (s == 2) a.out[(base + 3) * a.ld + p] = y;
     }

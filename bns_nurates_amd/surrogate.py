@@ -2,8 +2,12 @@
 (bns_nurates_amd/csrc/mlp.hip) trained on data the FP64 path generates, with
 per-point fallback to the FP64 path.
 
-Features of a point (12): log nb, log T, Ye, and for nue, anue, nux: log n,
-log J, chi (the heavy antineutrino copies nux).  Outputs (15): for nue,
+Features of a point (15): log nb, log T, Ye, and for nue, anue, nux: log n,
+log J, chi (the heavy antineutrino copies nux); then mu_e / T,
+(mu_n - mu_p) / T and 1 / T.  The last three are inputs of the opacity
+function itself (the chemical potentials of the point block): the rates
+depend on them through exponentials such as e^{-eta_e}, which a linear
+model in these features carries exactly.  Outputs (15): for nue,
 anue, nux the logarithms of eta_0, eta, kappa_0_a, kappa_a, kappa_s.  Both
 are standardised with statistics of the training set; the training box is
 the per-feature range of the training set.
@@ -17,21 +21,21 @@ import numpy as np
 from . import api
 from . import workloads as W
 
-FEAT, OUT = 12, 15
+FEAT, OUT = 15, 15
 # output block rows of the 15 outputs, species-major
 OUT_ROWS = [r for s in range(3) for r in (0 + s, 8 + s, 4 + s, 12 + s, 16 + s)]
 
 
 class MlpNorm(C.Structure):
-    _fields_ = [("feat_mean", C.c_float * 12), ("feat_std", C.c_float * 12),
-                ("out_mean", C.c_float * 15), ("out_std", C.c_float * 15),
-                ("box_lo", C.c_float * 12), ("box_hi", C.c_float * 12),
-                ("lin", (C.c_float * 15) * 13),
+    _fields_ = [("feat_mean", C.c_float * FEAT), ("feat_std", C.c_float * FEAT),
+                ("out_mean", C.c_float * OUT), ("out_std", C.c_float * OUT),
+                ("box_lo", C.c_float * FEAT), ("box_hi", C.c_float * FEAT),
+                ("lin", (C.c_float * OUT) * (FEAT + 1)),
                 ("res_lo", C.c_float * 15), ("res_hi", C.c_float * 15)]
 
 
 def features(pts: np.ndarray) -> np.ndarray:
-    """Raw features [npts, 12] (float64) of a point block [21, npts]; a zero
+    """Raw features [npts, 15] (float64) of a point block [21, npts]; a zero
     density gives -inf (outside any training box)."""
     f = np.empty((pts.shape[1], FEAT))
     f[:, 0] = np.log(pts[W.P_NB])
@@ -41,6 +45,10 @@ def features(pts: np.ndarray) -> np.ndarray:
         f[:, 3 + 3 * s] = np.log(pts[W.P_N0 + s])
         f[:, 4 + 3 * s] = np.log(pts[W.P_J0 + s])
         f[:, 5 + 3 * s] = pts[W.P_CHI0 + s]
+    rT = 1.0 / pts[W.P_T]
+    f[:, 12] = pts[W.P_MUE] * rT
+    f[:, 13] = (pts[W.P_MUN] - pts[W.P_MUP]) * rT
+    f[:, 14] = rT
     return f
 
 
@@ -141,14 +149,14 @@ class Ensemble:
             for i, x in enumerate(np.asarray(v, np.float64)):
                 arr[i] = float(x)
         lin = np.asarray(lin, np.float64)
-        for i in range(13):
-            for o in range(15):
+        for i in range(FEAT + 1):
+            for o in range(OUT):
                 n.lin[i][o] = float(lin[i, o])
         self._check(self.lib.bnsamd_mlp_set_norm(self.h, C.byref(n)), "bnsamd_mlp_set_norm")
         self.norm = n
 
     def gradients(self, member: int, x, y):
-        """Loss and gradient blob on rows of device tensors x [rows, 12], y [rows, 15]."""
+        """Loss and gradient blob on rows of device tensors x [rows, 15], y [rows, 15]."""
         g = np.empty(self.nparams, np.float32)
         loss = C.c_float()
         self._check(self.lib.bnsamd_mlp_gradients(self.h, member, x.shape[0], _fptr(x), _fptr(y),

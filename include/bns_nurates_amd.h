@@ -238,8 +238,9 @@ void bnsamd_host_free(void* ptr);
 
 /* ---- learned form: BF16-MFMA MLP ensemble ------------------------------------
  * No counterpart in the reference.  A small network per ensemble member maps
- * 12 features of a point (log nb, log T, Ye and, for nue, anue, nux, log n,
- * log J, chi) to 15 standardised logarithms of the outputs (eta_0, eta,
+ * 15 features of a point (log nb, log T, Ye and, for nue, anue, nux, log n,
+ * log J, chi; then the degeneracies mu_e / T and (mu_n - mu_p) / T, and
+ * 1 / T) to 15 standardised logarithms of the outputs (eta_0, eta,
  * kappa_0_a, kappa_a, kappa_s of nue, anue, nux); it is trained on the GPU by
  * the library's own kernels on data the FP64 path generates, and evaluated by
  * a fused kernel whose points fall back to the FP64 path when the members
@@ -252,10 +253,10 @@ typedef struct bnsamd_mlp bnsamd_mlp;
  * y is the network's output (the network learns the residual of a linear
  * model in the standardised features) */
 typedef struct bnsamd_mlp_norm {
-  float feat_mean[12], feat_std[12];
+  float feat_mean[15], feat_std[15];
   float out_mean[15], out_std[15];
-  float box_lo[12], box_hi[12];
-  float lin[13][15];
+  float box_lo[15], box_hi[15];
+  float lin[16][15];
   float res_lo[15], res_hi[15]; /* range of the standardised residual in training */
 } bnsamd_mlp_norm;
 
@@ -268,7 +269,7 @@ int bnsamd_mlp_set_params(bnsamd_mlp* m, int member, const float* h_params);
 int bnsamd_mlp_get_params(bnsamd_mlp* m, int member, float* h_params);
 int bnsamd_mlp_set_norm(bnsamd_mlp* m, const bnsamd_mlp_norm* norm);
 /* loss and gradient (host blob) of one member on `rows` rows of normalised
- * features d_x [rows][12] and standardised targets d_y [rows][15] */
+ * features d_x [rows][15] and standardised targets d_y [rows][15] */
 int bnsamd_mlp_gradients(bnsamd_mlp* m, int member, int rows, const float* d_x, const float* d_y,
                          float* h_grad, float* h_loss);
 /* Adam training of one member: `epochs` passes over n rows in the order

@@ -10,7 +10,7 @@ this restatement.
 """
 import numpy as np
 
-FEAT, OUT, K0, NL, HID, LAYERS = 12, 15, 32, 16, 256, 5
+FEAT, OUT, K0, NL, HID, LAYERS = 15, 15, 32, 16, 256, 5
 LAYER_K = [K0, HID, HID, HID, HID]
 LAYER_N = [HID, HID, HID, HID, NL]
 
@@ -48,7 +48,7 @@ def _sig(z):
 
 def forward(params, x, exact=False):
     """Pre-activations Z_l and activations H_l of one member on normalised
-    features x [rows, 12]: rounded as the kernels round (BF16 operands, FP32
+    features x [rows, 15]: rounded as the kernels round (BF16 operands, FP32
     results), or, with exact=True, in float64 throughout (the mathematical
     network, against which the gradient formulas are pinned)."""
     rnd = (lambda a: np.asarray(a, np.float64)) if exact else bf16

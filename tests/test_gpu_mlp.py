@@ -36,7 +36,7 @@ def test_forward_matches_oracle(ens):
     rng = np.random.default_rng(1)
     p = _params(2)
     ens.set_params(0, p)
-    x = rng.normal(size=(1000, 12)).astype(np.float32)
+    x = rng.normal(size=(1000, M.FEAT)).astype(np.float32)
     y = torch.empty((1000, 15), dtype=torch.float32, device="cuda")
     ens.predict(0, torch.from_numpy(x).cuda(), y)
     zs, _ = M.forward(p, x)
@@ -50,8 +50,8 @@ def test_gradients_match_oracle(ens):
     rng = np.random.default_rng(4)
     p = _params(5)
     ens.set_params(1, p)
-    x = rng.normal(size=(512, 12)).astype(np.float32)
-    y = rng.normal(size=(512, 15)).astype(np.float32)
+    x = rng.normal(size=(512, M.FEAT)).astype(np.float32)
+    y = rng.normal(size=(512, M.OUT)).astype(np.float32)
     loss, g = ens.gradients(1, torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
     lo, go = M.gradients(p, x, y)
     wo, bo, _ = M.offsets()
