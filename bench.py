@@ -266,6 +266,22 @@ def surrogate_eval(ctx, dpts, ref, fp64_ms, params, nquad, train_points, epochs,
            "final_epoch_loss": [float(f"{h[-1]:.4g}") for h in ens.history],
            "fp64_kernel_ms": fp64_ms, "thresholds": []}
     out = torch.empty_like(ref)
+    # the network alone, judged by the FP64 path: with an infinite spread
+    # threshold only points outside the training box (or whose predicted
+    # residual leaves its training range) fall back; the fraction of all
+    # points whose 15 predicted outputs are all within a relative error
+    ctx.set_quadrature(nquad)
+    nfb = ens.m1_opacities(ctx, dpts, out, float("inf"), W.FLAGS_ALL, params)
+    o = out.cpu().numpy()
+    pred = ~np.all(o == r, axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        worst = np.max(np.where(o[rows] == r[rows], 0.0,
+                                np.abs(o[rows] - r[rows]) / np.abs(r[rows])), axis=0)
+    res["network_alone"] = {
+        "predicted_fraction": round(float(pred.mean()), 4),
+        "all_outputs_within": {str(tol): round(float((pred & (worst <= tol)).mean()), 4)
+                               for tol in (1e-2, 3e-2, 1e-1)}}
+    log(f"surrogate network alone: {res['network_alone']}")
     for thr in thresholds:
         ctx.set_quadrature(nquad)
         nfb = ens.m1_opacities(ctx, dpts, out, thr, W.FLAGS_ALL, params)  # warm-up

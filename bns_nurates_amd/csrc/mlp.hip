@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/bns_nurates_amd.h"
+#include "gray_math.h"
 #include "mlp.h"
 
 namespace bns {
@@ -433,7 +434,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
   __syncthreads();
   // outputs of the accepted points: column c = 5 s + q (q: eta_0, eta,
-  // kappa_0_a, kappa_a, kappa_s) goes to field 4 q' + s, anux copying nux
+  // kappa_0_a, kappa_a, kappa_s) goes to field 4 q' + s, anux copying nux.
+  // The opacities keep the zeros the FP64 path gives below the density
+  // thresholds (m1_opacities.hpp:1199-1292): kappa_0_a needs n > 1e-21,
+  // kappa_a and kappa_s need J > 1e-25, and kappa_a of nue is zero when n is
+  // (for anux, its own n and J decide).
   const int64_t p = p0 + r;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -445,8 +450,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
       for (int i = 0; i < kFeat; ++i) lin = fmaf(nm.lin[1 + i][c], xs[r][i], lin);
       const double y = exp(double(lin) + double(mean[v] * nm.out_std[c] + nm.out_mean[c]));
-      a.out[(base + s) * a.ld + p] = y;
-      if (s == 2) a.out[(base + 3) * a.ld + p] = y;
+      auto value = [&](int sp) {
+        const double ns = a.pts[(9 + sp) * a.ld + p], Js = a.pts[(13 + sp) * a.ld + p];
+        if (q == 2) return ns > bns::c::ThresholdN ? y : 0.;
+        if (q >= 3) return (Js > bns::c::ThresholdJ && !(q == 3 && sp == 0 && ns == 0.)) ? y : 0.;
+        return y;
+      };
+      a.out[(base + s) * a.ld + p] = value(s);
+      if (s == 2) a.out[(base + 3) * a.ld + p] = value(3);
     }
   }
 }

@@ -4,7 +4,8 @@ kernel, and the per-point fallback to the FP64 path judged against it.
 
 * points the ensemble does not trust (spread above the threshold, or
   features outside the training box) carry the FP64 path's bits exactly;
-* accepted points stay within the stated tolerance of the FP64 path;
+* accepted points stay within the stated tolerance of the FP64 path, and
+  keep the exact zeros the FP64 path gives below the n / J thresholds;
 * the fused kernel agrees with the training-path forward pass of each
   member (ensemble mean);
 * out-of-box inputs always fall back.
@@ -49,7 +50,9 @@ def test_fallback_and_accepted_points(trained):
     torch.cuda.synchronize()
     o, r = out.cpu().numpy(), ref.cpu().numpy()
     rows = S.OUT_ROWS + [3, 7, 11, 15, 19]
-    rel = np.abs(o[rows] - r[rows]) / np.abs(r[rows])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = np.where(r[rows] != 0, np.abs(o[rows] - r[rows]) / np.abs(r[rows]),
+                       np.where(o[rows] == 0, 0.0, np.inf))
     same = np.all(o == r, axis=0)
     acc = ~same
     say(f"surrogate: {nfb} of {pts.shape[1]} points fell back ({nfb / pts.shape[1]:.1%}); "
@@ -61,6 +64,54 @@ def test_fallback_and_accepted_points(trained):
     # spread is a proxy: single accepted outputs can be off by more)
     assert acc.sum() > 0
     assert np.median(rel[:, acc]) < THR
+    # the threshold zeros of the opacities (n <= 1e-21, J <= 1e-25)
+    zero = r == 0
+    say(f"surrogate: {int(zero[:, acc].sum())} exact-zero outputs on accepted points")
+    assert np.array_equal(o[zero], r[zero])
+
+
+def test_threshold_zeros_on_accepted_points(trained):
+    """Accepted points below the reference's n / J thresholds
+    (m1_opacities.hpp:1199-1292) get exact zeros for kappa_0_a / kappa_a /
+    kappa_s, as the FP64 path gives.  The trained box excludes such points
+    (their targets have no logarithm), so the box and the residual range are
+    opened here: every point is then accepted and the zeros must come from
+    the kernel's own threshold rule."""
+    ctx, ens, prm = trained
+    pts = W.make_points(4096, seed=80)
+    rng = np.random.default_rng(3)
+    for s in range(3):
+        for fld, thr in ((W.P_N0 + s, 1e-21), (W.P_J0 + s, 1e-25)):
+            idx = rng.choice(pts.shape[1], 64, replace=False)
+            pts[fld, idx] = thr * rng.choice([0.0, 0.5, 1.0], 64)
+            if s == 2:
+                pts[fld + 1, idx] = pts[fld, idx]  # anux copies nux
+    nm = ens.norm
+    keep = {k: np.array(getattr(nm, k)[:]) for k in
+            ("feat_mean", "feat_std", "out_mean", "out_std", "box_lo", "box_hi", "res_lo", "res_hi")}
+    lin = np.array([[nm.lin[i][o] for o in range(S.OUT)] for i in range(S.FEAT + 1)])
+    big = np.full(S.FEAT, 1e30)
+    wide = np.full(S.OUT, 1e30)
+    d = torch.from_numpy(pts).cuda()
+    ctx.set_quadrature(NQ)
+    ref = ctx.compute_m1_opacities(d, flags=W.FLAGS_ALL, params=prm).cpu().numpy()
+    out = torch.full((20, pts.shape[1]), float("nan"), dtype=torch.float64, device="cuda")
+    try:
+        ens.set_norm(keep["feat_mean"], keep["feat_std"], keep["out_mean"], keep["out_std"],
+                     -big, big, lin, -wide, wide)
+        nfb = ens.m1_opacities(ctx, d, out, 1e30, W.FLAGS_ALL, prm)
+        torch.cuda.synchronize()
+    finally:
+        ens.set_norm(keep["feat_mean"], keep["feat_std"], keep["out_mean"], keep["out_std"],
+                     keep["box_lo"], keep["box_hi"], lin, keep["res_lo"], keep["res_hi"])
+    o = out.cpu().numpy()
+    zero = ref == 0
+    say(f"threshold zeros: {int(zero.sum())} zero outputs, {nfb} fallback points")
+    # log(0) features fall outside any box; the points with n or J exactly 0
+    # fall back, the rest are the network's with the kernel's zeros
+    assert nfb < pts.shape[1] // 2
+    assert zero.sum() > 0
+    assert np.array_equal(o[zero], ref[zero])
 
 
 def test_fused_kernel_matches_training_path(trained):
@@ -90,7 +141,10 @@ def test_fused_kernel_matches_training_path(trained):
     inbox = np.all((f >= np.array(nm.box_lo)) & (f <= np.array(nm.box_hi)), axis=1)
     use = inbox & ~fell
     # in the network's own (standardised) units: BF16 rounding noise only
-    err = (np.abs(np.log(o[use]) - mean[use]) / ts).max()
+    # (outputs set to zero by the n / J thresholds are not the network's)
+    pos = o[use] > 0
+    with np.errstate(divide="ignore"):
+        err = np.where(pos, np.abs(np.log(o[use]) - mean[use]) / ts, 0.0).max()
     say(f"fused inference vs training-path forward: max |d y| = {err:.3e} on {use.sum()} points")
     assert use.sum() > 0 and err < 5e-2
 
