@@ -33,7 +33,10 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
   __shared__ double s_Fp[5][64];  // F_1..F_5 (eta + y_a)
   __shared__ double s_g[4][64];   // TotalNuF(nu_a, s)
-  __shared__ double s_bf[4][64];  // blocking factor 1 - g (or 1)
+  // blocking factor 1 - g of the mixed form (formed without cancellation);
+  // the FP64 form takes 1 - g from s_g, the same bits, so that its LDS stays
+  // below 10 KB (four workgroups of one wave per SIMD)
+  __shared__ double s_bf[MIX && BLOCK ? 4 : 1][64];
   __shared__ double s_y[64];      // node energies / T, formed as the reference does
   // wave-uniform per-point data, read back with broadcast LDS loads so that
   // it occupies neither scalar nor vector registers through the cell loops
@@ -240,7 +243,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
             go.omg = 1. - go.g;
           }
           s_g[s][a] = go.g;
-          s_bf[s][a] = blocking ? go.omg : 1.;
+          if constexpr (MIX && BLOCK) s_bf[s][a] = go.omg;
         }
       }
     }
@@ -424,7 +427,9 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           double ke = (ord == 0 ? Kem[s] : Kem[ps]) + bem;
           double ka = (ord == 0 ? Kab[s] : Kab[ps]) + bab;
           double gu = s_g[s][u];
-          double eb = ke * s_bf[ps][v];
+          double bfv = 1.;
+          if constexpr (BLOCK) bfv = MIX ? s_bf[ps][v] : 1. - s_g[ps][v];
+          double eb = ke * bfv;
           double e_ent, a_ent;
           if constexpr (stim) {
             e_ent = eb;
@@ -677,14 +682,14 @@ __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_
 
 }  // namespace bns
 
-#ifndef BNS_GRAY_WAVES_PER_EU
-#define BNS_GRAY_WAVES_PER_EU 3
-#endif
 namespace bns {
 namespace {
 template <bool MIX, bool STIM, bool BLOCK>
 void launch_instance(const GrayKernelArgs& a, dim3 grid, hipStream_t stream) {
-  constexpr int W = BNS_GRAY_WAVES_PER_EU;
+  // 4 waves per SIMD (128 VGPRs; the spills are a few values reloaded once
+  // per cell round); the mixed form with blocking keeps its 2 KB blocking
+  // table, and its LDS allows 3.7 workgroups per SIMD, so it stays at 3
+  constexpr int W = (MIX && BLOCK) ? 3 : 4;
   if constexpr (MIX)
     hipLaunchKernelGGL((k_gray_m1_mixed<W, STIM, BLOCK>), grid, dim3(64), 0, stream, a);
   else
