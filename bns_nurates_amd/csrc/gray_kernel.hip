@@ -22,6 +22,10 @@
 
 namespace bns {
 
+// slot of eta = mu_e / T in the per-point LDS block (after the 21 inputs and
+// 28 distribution values, before log T and 1 / T)
+constexpr int kEtaSlot = 61;
+
 // ---------------------------------------------------------------------------
 // The body of both kernel variants: MIX = false is the FP64 path, MIX = true
 // the mixed-precision path (FP32 distribution values and HR98 channel
@@ -42,6 +46,10 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   // it occupies neither scalar nor vector registers through the cell loops
   __shared__ double s_in[64];     // the 21 inputs (+ 28 distribution values), 1 / T, log T
   __shared__ Spec s_D[4];
+  // the 1D results (iso[4]; beta n_em, e_em, n_ab, e_ab of nue, anue), parked
+  // here through the 2D phase so that they hold no registers there
+  __shared__ double s_r1d[12];
+  __shared__ double s_brem[6];
   __shared__ double s_Fe[2][6], s_fd0[2];
   __shared__ HrChan s_hc[3];
 
@@ -74,6 +82,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   // the logarithms of the NEPS energies are log(T) + log(nu / T)
   if (lane == 63) s_in[kInvT] = 1. / T;
   if (lane == 62) s_in[kLogT] = log(T);
+  if (lane == 61) s_in[kEtaSlot] = mu_e / T;
 
   // ---- per-point constants --------------------------------------------------
   const double eta = mu_e / T;
@@ -111,7 +120,8 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   for (int i = 0; i < 16; ++i) acc[i] = 0.;
 
   // ======================= 1D: isoenergetic scattering =======================
-  double iso[4] = {0., 0., 0., 0.};
+  const volatile double* vr1d = s_r1d;
+  if (lane < 12) s_r1d[lane] = 0.;
   const double s_nux = 1.5 * T;
   double s_iso[4];
 #pragma unroll
@@ -141,11 +151,13 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       a4[3] += s == 3 ? v : 0.;
     }
 #pragma unroll
-    for (int s = 0; s < 4; ++s) iso[s] = wave_sum(a4[s]) * s_iso[s];
+    for (int s = 0; s < 4; ++s) {
+      const double v = wave_sum(a4[s]) * s_iso[s];
+      if (lane == 0) s_r1d[s] = v;
+    }
   }
 
   // ========================== 1D: beta processes ============================
-  double bne[2] = {0., 0.}, bje[2] = {0., 0.}, bna[2] = {0., 0.}, bja[2] = {0., 0.};
   double s_beta[2];
   if (eta > -30. && eta < 30.) {
     s_beta[0] = T * vFe[0][5] / vFe[0][4];
@@ -187,10 +199,14 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     }
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      bne[s] = wave_sum(a8[0 + s]) * s_beta[s];
-      bje[s] = wave_sum(a8[2 + s]) * s_beta[s];
-      bna[s] = wave_sum(a8[4 + s]) * s_beta[s];
-      bja[s] = wave_sum(a8[6 + s]) * s_beta[s];
+      const double ne = wave_sum(a8[0 + s]) * s_beta[s], je = wave_sum(a8[2 + s]) * s_beta[s];
+      const double na = wave_sum(a8[4 + s]) * s_beta[s], ja = wave_sum(a8[6 + s]) * s_beta[s];
+      if (lane == 0) {
+        s_r1d[4 + s] = ne;
+        s_r1d[6 + s] = je;
+        s_r1d[8 + s] = na;
+        s_r1d[10 + s] = ja;
+      }
     }
   }
 
@@ -249,7 +265,9 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     }
     __syncthreads();
 
-    // per-point bremsstrahlung constants
+    // per-point bremsstrahlung constants, kept in LDS (s_brem: BRT06
+    // prefactor; HR98 y, e^-y, p, 1/p, medium factor) and read in the cell
+    // loop where they are used
     double brt_pre = 0.;
     double hr_y = 0., hr_emy = 0., hr_p = 1., hr_pinv = 1., hr_medium = 1.;
     if (do_brem && A.params.brem_impl == 1) {
@@ -273,12 +291,22 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       const HrChan hc = hr98_channel(lane < 3 ? lane : 0, T, vin[0], vin[2], vin[3], hp.y);
       if (lane < 3) s_hc[lane] = hc;
     }
+    if (lane == 0) {
+      s_brem[0] = brt_pre;
+      s_brem[1] = hr_y;
+      s_brem[2] = hr_emy;
+      s_brem[3] = hr_p;
+      s_brem[4] = hr_pinv;
+      s_brem[5] = hr_medium;
+    }
 
     __syncthreads();
     const double a1e = c::PairA1E * c::PairA1E, a2e = c::PairA2E * c::PairA2E;
     const double a1x = c::PairA1X * c::PairA1X, a2x = c::PairA2X * c::PairA2X;
-    const double pairT2 = c::PairPhiC * (T * T);
+    const volatile double* vbrem = s_brem;
     for (int cc = lane; cc < L.P; cc += kWave) {
+      const double eta = vin[kEtaSlot];
+      const double pairT2 = c::PairPhiC * (vin[1] * vin[1]);
       const double* __restrict__ cr = tb + L.cl + CL_REC * cc;
       int ab = int(cr[CL_AB]);
       int a = ab >> 8, b = ab & 255;
@@ -340,6 +368,8 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       }
       double bem = 0., bab = 0.;
       if (do_brem) {
+        const double brt_pre = vbrem[0], hr_y = vbrem[1], hr_emy = vbrem[2];
+        const double hr_p = vbrem[3], hr_pinv = vbrem[4], hr_medium = vbrem[5];
         if (A.params.brem_impl == 1) {
           bem = brt_pre * cr[CL_K1EM];
           bab = brt_pre * cr[CL_K1AB];
@@ -449,11 +479,15 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
 
   // ====================== 2D: inelastic scattering on e-+ ===================
   if (A.flags.use_inelastic_scatt == 1) {
-    const double pre = c::NEPSConst * (T * T);
     const double bpl = c::NEPSBPlus, bze = c::NEPSBZero, bmi = c::NEPSBMinus;
-    // t = 4 s_neps with s_neps = 2.182 T (m1_opacities.hpp:1092, 1167)
-    const double ht = 0.5 * (4. * (c::PairScale * T));
     for (int h = lane; h < L.Q; h += kWave) {
+      // T and eta from LDS in every round, and the products formed from them
+      // here: hoisted out of the loop they would hold registers through it
+      // (spilled at four waves per SIMD)
+      const double T = vin[1], eta = vin[kEtaSlot];
+      const double pre = c::NEPSConst * (T * T);
+      // t = 4 s_neps with s_neps = 2.182 T (m1_opacities.hpp:1092, 1167)
+      const double ht = 0.5 * (4. * (c::PairScale * T));
       const double* __restrict__ hr = tb + L.np + NP_REC * h;
       // nu, nubar and w = nu / T with the reference's roundings
       // (m1_opacities.hpp:910-912, 982-984; kernel_nes.hpp:144-145) so that
@@ -616,6 +650,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = wave_sum(acc[i]);
   if (lane == 0) {
+    const double T = vin[1];
     const double tp = c::PairScale * T;
     const double T2 = T * T;
     const double sN = tp * tp * (T2 * T2);  // t^2 T^4
@@ -635,14 +670,15 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     // m1_opacities.hpp:1187-1292
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      o[0 + s] = F * (F * n2d[s] + bne[s]);
-      o[8 + s] = F * (F * e2d[s] + bje[s]);
+      const double bne = vr1d[4 + s], bje = vr1d[6 + s], bna = vr1d[8 + s], bja = vr1d[10 + s];
+      o[0 + s] = F * (F * n2d[s] + bne);
+      o[8 + s] = F * (F * e2d[s] + bje);
       const double ns = vin[9 + s], Js = vin[13 + s];
-      if (ns > c::ThresholdN) o[4 + s] = F / (cl * ns) * (F * n2d[4 + s] + bna[s]);
+      if (ns > c::ThresholdN) o[4 + s] = F / (cl * ns) * (F * n2d[4 + s] + bna);
       if (Js > c::ThresholdJ) {
-        double ka = F / (cl * Js) * (F * e2d[4 + s] + bja[s]);
+        double ka = F / (cl * Js) * (F * e2d[4 + s] + bja);
         o[12 + s] = (s == 0 && ns == 0.) ? 0. : ka;
-        o[16 + s] = F / (cl * Js) * iso[s];
+        o[16 + s] = F / (cl * Js) * vr1d[s];
       }
     }
 #pragma unroll
@@ -656,7 +692,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       }
       if (Js > c::ThresholdJ) {
         o[12 + s] = F2 / (cl * Js) * e2d[4 + s];
-        o[16 + s] = F / (cl * Js) * iso[s];
+        o[16 + s] = F / (cl * Js) * vr1d[s];
       }
     }
 #pragma unroll
