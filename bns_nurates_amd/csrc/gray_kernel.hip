@@ -413,13 +413,16 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           double x = cr[CL_S];
           x = x > c::BremMin ? x : c::BremMin;
           double y = hr_y > c::BremMin ? hr_y : c::BremMin;
-          double snum = 2. * c::SqrtPi * exp(1.5 * log(x + 2. - hr_emy)) *
+          // the exponents below are logarithms of O(1e-30..1e30) quantities
+          // scaled by O(1) powers: the bounded exponential (within an ulp of
+          // e^x) without the library's range handling
+          double snum = 2. * c::SqrtPi * exp_bounded(1.5 * log(x + 2. - hr_emy)) *
                         (x * x + 2. * x * y + 5. * (y * y) / 3. + 1.);
           double bb = c::Pi2OneEighth + x + y;
           double b2 = bb * bb;
           double snd = div_normal(snum, c::SqrtPi + b2 * b2);
           double lsnd = log(snd), lsdx = cr[CL_LSD];
-          double pnd = exp(-hr_p * lsnd);
+          double pnd = exp_bounded(-hr_p * lsnd);
           double px11 = cr[CL_X11], px093 = cr[CL_X093];
           double px12 = cr[CL_X12], px28 = cr[CL_X28];
           double xm = x - 1.2;
@@ -428,8 +431,8 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
             const volatile HrChan& h = s_hc[ch];
-            double sd = exp(-hr_p * (h.lsd_pt + lsdx));
-            double sbase = exp(hr_pinv * log(pnd + sd));
+            double sd = exp_bounded(-hr_p * (h.lsd_pt + lsdx));
+            double sbase = exp_bounded(hr_pinv * log(pnd + sd));
             // the operands are normal and the quotients O(1e-30..1e30)
             double fb = 1. + div_normal(1., fx * h.f_pt);
             double gbm = 1. - px11 * h.g0;
