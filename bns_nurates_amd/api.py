@@ -202,6 +202,8 @@ def load_library():
         "bnsamd_mlp_gradients": ([vp, C.c_int, C.c_int, vp, vp, vp, vp], C.c_int),
         "bnsamd_mlp_train": ([vp, C.c_int, i64, vp, vp, vp, C.c_int, C.c_int, C.c_float,
                               C.c_float, vp], C.c_int),
+        "bnsamd_mlp_train_ensemble": ([vp, i64, vp, vp, vp, C.c_int, C.c_int, C.c_float,
+                                       C.c_float, vp], C.c_int),
         "bnsamd_mlp_predict": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
         "bnsamd_surrogate_m1_opacities": ([vp, vp, i64, i64, vp, C.c_float, C.POINTER(_Flags),
                                            C.POINTER(_Params), C.c_int, vp, C.POINTER(i64)],

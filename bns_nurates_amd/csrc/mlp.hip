@@ -481,6 +481,23 @@ __global__ void k_scatter_out(int32_t n, const int32_t* __restrict__ idx, const 
 }  // namespace mlp
 }  // namespace bns
 
+namespace bns {
+namespace mlp {
+// training workspace of one stream: maxb rows of every layer's activations
+struct Work {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint16_t* Xb = nullptr;
+  float* Tb = nullptr;
+  float* Z[kLayers] = {};
+  uint16_t* H[kLayers] = {};
+  uint16_t* dZ[kLayers] = {};
+  float* dH = nullptr;
+  float* loss = nullptr;
+};
+}  // namespace mlp
+}  // namespace bns
+
 using namespace bns::mlp;
 
 // ---- the MLP object -------------------------------------------------------------
@@ -495,14 +512,10 @@ struct bnsamd_mlp {
   std::vector<int64_t> steps;
   float* norm = nullptr;  // device copy of the normalisation block
   bnsamd_mlp_norm hnorm{};
-  // training workspace (maxb rows)
-  uint16_t* Xb = nullptr;
-  float* Tb = nullptr;
-  float* Z[kLayers] = {};
-  uint16_t* H[kLayers] = {};
-  uint16_t* dZ[kLayers] = {};
-  float* dH = nullptr;
-  float* loss = nullptr;
+  // training workspaces: ws[0] on the main stream (single-member training,
+  // gradients, prediction); ws[e] for member e of bnsamd_mlp_train_ensemble,
+  // on its own stream, allocated on first use
+  std::vector<Work> ws;
   // fallback workspace (grown on demand)
   int64_t fb_cap = 0;
   int32_t* fb_idx = nullptr;
@@ -537,49 +550,94 @@ void fill(hipStream_t s, int64_t n, float* p, float v) {
   hipLaunchKernelGGL(k_fill, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, p, v);
 }
 
-// forward pass of one member on B rows of Xb (BF16 [B][32]); Z[l] hold the
-// pre-activations, H[l] the BF16 activations
-void forward(bnsamd_mlp* m, int e, int B) {
-  hipStream_t s = m->stream;
-  const uint16_t* in = m->Xb;
+// forward pass of member e on B rows of W.Xb (BF16 [B][32]); W.Z[l] hold the
+// pre-activations, W.H[l] the BF16 activations
+void forward(bnsamd_mlp* m, Work& W, int e, int B) {
+  hipStream_t s = W.stream;
+  const uint16_t* in = W.Xb;
   for (int l = 0; l < kLayers; ++l) {
     const int K = kLayerK[l], N = kLayerN[l];
-    gemm(s, B, N, K, in, K, 1, m->wb[e] + kWOff[l], N, 1, m->Z[l], N, 1);
-    hipLaunchKernelGGL(k_bias_act, dim3((B + 63) / 64), dim3(256), 0, s, B, N, m->Z[l],
+    gemm(s, B, N, K, in, K, 1, m->wb[e] + kWOff[l], N, 1, W.Z[l], N, 1);
+    hipLaunchKernelGGL(k_bias_act, dim3((B + 63) / 64), dim3(256), 0, s, B, N, W.Z[l],
                        m->w[e] + kBOff[l], l + 1 < kLayers ? 1 : 0,
-                       l + 1 < kLayers ? m->H[l] : static_cast<uint16_t*>(nullptr));
-    in = m->H[l];
+                       l + 1 < kLayers ? W.H[l] : static_cast<uint16_t*>(nullptr));
+    in = W.H[l];
   }
 }
 
-// loss, gradients of one member on B rows (Xb, Tb) into g[e]
-void backward(bnsamd_mlp* m, int e, int B) {
-  hipStream_t s = m->stream;
+// loss, gradients of member e on B rows (W.Xb, W.Tb) into g[e]
+void backward(bnsamd_mlp* m, Work& W, int e, int B) {
+  hipStream_t s = W.stream;
   fill(s, kParams, m->g[e], 0.f);
   const int L = kLayers - 1;
-  hipLaunchKernelGGL(k_mse_grad, dim3((B + 255) / 256), dim3(256), 0, s, B, kNL, kOut, m->Z[L],
-                     m->Tb, m->dZ[L], m->g[e] + kBOff[L], m->loss);
+  hipLaunchKernelGGL(k_mse_grad, dim3((B + 255) / 256), dim3(256), 0, s, B, kNL, kOut, W.Z[L],
+                     W.Tb, W.dZ[L], m->g[e] + kBOff[L], W.loss);
   for (int l = L; l >= 0; --l) {
     const int K = kLayerK[l], N = kLayerN[l];
-    const uint16_t* Hin = l ? m->H[l - 1] : m->Xb;
+    const uint16_t* Hin = l ? W.H[l - 1] : W.Xb;
     // dW = H_in^T dZ (reduction over the batch, split into 512-row chunks)
-    gemm(s, K, N, B, Hin, 1, K, m->dZ[l], N, 1, m->g[e] + kWOff[l], N, (B + 511) / 512);
+    gemm(s, K, N, B, Hin, 1, K, W.dZ[l], N, 1, m->g[e] + kWOff[l], N, (B + 511) / 512);
     if (l) {
       // dH_in = dZ W^T, then through the SiLU of layer l - 1
-      gemm(s, B, K, N, m->dZ[l], N, 1, m->wb[e] + kWOff[l], 1, N, m->dH, K, 1);
-      hipLaunchKernelGGL(k_act_bwd, dim3((B + 63) / 64), dim3(256), 0, s, B, K, m->dH,
-                         m->Z[l - 1], m->dZ[l - 1], m->g[e] + kBOff[l - 1]);
+      gemm(s, B, K, N, W.dZ[l], N, 1, m->wb[e] + kWOff[l], 1, N, W.dH, K, 1);
+      hipLaunchKernelGGL(k_act_bwd, dim3((B + 63) / 64), dim3(256), 0, s, B, K, W.dH,
+                         W.Z[l - 1], W.dZ[l - 1], m->g[e] + kBOff[l - 1]);
     }
   }
 }
 
-void adam(bnsamd_mlp* m, int e, float lr) {
+void adam(bnsamd_mlp* m, hipStream_t s, int e, float lr) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   int64_t t = ++m->steps[e];
   float c1 = 1.f / (1.f - std::pow(b1, float(t))), c2 = 1.f / (1.f - std::pow(b2, float(t)));
-  hipLaunchKernelGGL(k_adam, dim3(unsigned((kParams + 255) / 256)), dim3(256), 0, m->stream,
+  hipLaunchKernelGGL(k_adam, dim3(unsigned((kParams + 255) / 256)), dim3(256), 0, s,
                      int64_t(kParams), m->w[e], m->g[e], m->m[e], m->v[e], m->wb[e], lr, b1, b2,
                      eps, c1, c2);
+}
+
+// buffers of one workspace (maxb rows); false when an allocation fails
+bool work_alloc(Work& W, int maxb) {
+  bool ok = true;
+  auto alloc = [&](void** p, size_t bytes) {
+    ok = ok && hipMalloc(p, bytes) == hipSuccess && hipMemset(*p, 0, bytes) == hipSuccess;
+  };
+  const size_t B = size_t(maxb);
+  alloc(reinterpret_cast<void**>(&W.Xb), sizeof(uint16_t) * B * kK0);
+  alloc(reinterpret_cast<void**>(&W.Tb), sizeof(float) * B * kNL);
+  for (int l = 0; l < kLayers; ++l) {
+    alloc(reinterpret_cast<void**>(&W.Z[l]), sizeof(float) * B * kLayerN[l]);
+    alloc(reinterpret_cast<void**>(&W.H[l]), sizeof(uint16_t) * B * kLayerN[l]);
+    alloc(reinterpret_cast<void**>(&W.dZ[l]), sizeof(uint16_t) * B * kLayerN[l]);
+  }
+  alloc(reinterpret_cast<void**>(&W.dH), sizeof(float) * B * kHidden);
+  alloc(reinterpret_cast<void**>(&W.loss), sizeof(float));
+  return ok;
+}
+
+void work_free(Work& W) {
+  auto fr = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  if (W.stream) (void)hipStreamSynchronize(W.stream);
+  fr(W.Xb);
+  fr(W.Tb);
+  for (int l = 0; l < kLayers; ++l) {
+    fr(W.Z[l]);
+    fr(W.H[l]);
+    fr(W.dZ[l]);
+  }
+  fr(W.dH);
+  fr(W.loss);
+  if (W.own_stream && W.stream) (void)hipStreamDestroy(W.stream);
+  W = Work{};
+}
+
+// every stream that training may still be using
+hipError_t sync_all(bnsamd_mlp* m) {
+  hipError_t e = hipSuccess;
+  for (auto& W : m->ws)
+    if (W.stream && e == hipSuccess) e = hipStreamSynchronize(W.stream);
+  return e;
 }
 
 }  // namespace
@@ -616,17 +674,10 @@ int bnsamd_mlp_create(int device, int members, int max_batch, bnsamd_mlp** out) 
     m->wp.push_back(wp);
     m->steps.push_back(0);
   }
-  const size_t B = size_t(max_batch);
   alloc(reinterpret_cast<void**>(&m->norm), sizeof(bnsamd_mlp_norm));
-  alloc(reinterpret_cast<void**>(&m->Xb), sizeof(uint16_t) * B * kK0);
-  alloc(reinterpret_cast<void**>(&m->Tb), sizeof(float) * B * kNL);
-  for (int l = 0; l < kLayers; ++l) {
-    alloc(reinterpret_cast<void**>(&m->Z[l]), sizeof(float) * B * kLayerN[l]);
-    alloc(reinterpret_cast<void**>(&m->H[l]), sizeof(uint16_t) * B * kLayerN[l]);
-    alloc(reinterpret_cast<void**>(&m->dZ[l]), sizeof(uint16_t) * B * kLayerN[l]);
-  }
-  alloc(reinterpret_cast<void**>(&m->dH), sizeof(float) * B * kHidden);
-  alloc(reinterpret_cast<void**>(&m->loss), sizeof(float));
+  m->ws.resize(1);
+  m->ws[0].stream = m->stream;
+  ok = ok && work_alloc(m->ws[0], max_batch);
   if (!ok) {
     bnsamd_mlp_destroy(m);
     return BNSAMD_ERR_NOMEM;
@@ -638,7 +689,7 @@ int bnsamd_mlp_create(int device, int members, int max_batch, bnsamd_mlp** out) 
 void bnsamd_mlp_destroy(bnsamd_mlp* m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
-  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  (void)sync_all(m);
   auto fr = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -649,15 +700,7 @@ void bnsamd_mlp_destroy(bnsamd_mlp* m) {
   for (auto* p : m->wb) fr(p);
   for (auto* p : m->wp) fr(p);
   fr(m->norm);
-  fr(m->Xb);
-  fr(m->Tb);
-  for (int l = 0; l < kLayers; ++l) {
-    fr(m->Z[l]);
-    fr(m->H[l]);
-    fr(m->dZ[l]);
-  }
-  fr(m->dH);
-  fr(m->loss);
+  for (auto& W : m->ws) work_free(W);
   fr(m->fb_idx);
   fr(m->fb_count);
   fr(m->fb_pts);
@@ -690,7 +733,7 @@ int bnsamd_mlp_set_params(bnsamd_mlp* m, int member, const float* h_params) {
 int bnsamd_mlp_get_params(bnsamd_mlp* m, int member, float* h_params) {
   if (!m || member < 0 || member >= m->members || !h_params) return BNSAMD_ERR_INVALID;
   (void)hipSetDevice(m->device);
-  if (hipStreamSynchronize(m->stream) != hipSuccess ||
+  if (sync_all(m) != hipSuccess ||
       hipMemcpy(h_params, m->w[member], sizeof(float) * kParams, hipMemcpyDeviceToHost) !=
           hipSuccess)
     return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_get_params: copy");
@@ -718,21 +761,47 @@ int bnsamd_mlp_gradients(bnsamd_mlp* m, int member, int rows, const float* d_x, 
   if (hipMalloc(&perm, sizeof(int32_t) * rows) != hipSuccess)
     return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_gradients: hipMalloc");
   hipError_t e = hipMemcpy(perm, ident.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice);
-  fill(m->stream, 1, m->loss, 0.f);
-  hipLaunchKernelGGL(k_gather_batch, dim3((rows + 7) / 8), dim3(32, 8), 0, m->stream, rows, perm,
-                     int64_t(0), d_x, d_y, m->Xb, m->Tb);
-  forward(m, member, rows);
-  backward(m, member, rows);
+  Work& W = m->ws[0];
+  if (e == hipSuccess) e = sync_all(m);  // no member is still training
+  fill(W.stream, 1, W.loss, 0.f);
+  hipLaunchKernelGGL(k_gather_batch, dim3((rows + 7) / 8), dim3(32, 8), 0, W.stream, rows, perm,
+                     int64_t(0), d_x, d_y, W.Xb, W.Tb);
+  forward(m, W, member, rows);
+  backward(m, W, member, rows);
   float loss = 0.f;
-  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(W.stream);
   if (e == hipSuccess)
     e = hipMemcpy(h_grad, m->g[member], sizeof(float) * kParams, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(&loss, m->loss, sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&loss, W.loss, sizeof(float), hipMemcpyDeviceToHost);
   (void)hipFree(perm);
   if (e != hipSuccess) return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_gradients");
   if (h_loss) *h_loss = loss / float(rows * kOut);
   return BNSAMD_OK;
 }
+
+namespace {
+// one Adam step of member e on workspace W: batch rows of d_x / d_y in the
+// order d_perm from `start`
+void train_step(bnsamd_mlp* m, Work& W, int e, int batch, int64_t start, const float* d_x,
+                const float* d_y, const int32_t* d_perm, float lr) {
+  hipLaunchKernelGGL(k_gather_batch, dim3((batch + 7) / 8), dim3(32, 8), 0, W.stream, batch,
+                     d_perm, start, d_x, d_y, W.Xb, W.Tb);
+  forward(m, W, e, batch);
+  backward(m, W, e, batch);
+  adam(m, W.stream, e, lr);
+}
+
+// cosine decay from lr0 to lr1 over `total` steps
+float cosine_lr(float lr0, float lr1, int64_t k, int64_t total) {
+  return lr1 + 0.5f * (lr0 - lr1) * (1.f + std::cos(3.14159265f * float(k) / float(total)));
+}
+
+// row offset of step st of epoch ep: each epoch reads the permutation from a
+// rotating offset
+int64_t batch_start(int64_t st, int ep, int batch, int64_t n) {
+  return (st * batch + int64_t(ep) * 7919 * batch) % (n - batch + 1);
+}
+}  // namespace
 
 int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, const float* d_y,
                      const int32_t* d_perm, int epochs, int batch, float lr0, float lr1,
@@ -741,31 +810,73 @@ int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, con
       !d_x || !d_y || !d_perm || epochs < 1)
     return BNSAMD_ERR_INVALID;
   (void)hipSetDevice(m->device);
+  if (sync_all(m) != hipSuccess) return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train: sync");
   m->packed_valid = false;
+  Work& W = m->ws[0];
   const int64_t steps_per_epoch = n / batch, total = steps_per_epoch * epochs;
   int64_t k = 0;
   for (int ep = 0; ep < epochs; ++ep) {
-    fill(m->stream, 1, m->loss, 0.f);
-    for (int64_t st = 0; st < steps_per_epoch; ++st, ++k) {
-      // cosine decay from lr0 to lr1
-      float lr = lr1 + 0.5f * (lr0 - lr1) * (1.f + std::cos(3.14159265f * float(k) / float(total)));
-      // epoch ep reads the permutation from a rotating offset
-      int64_t start = (st * batch + int64_t(ep) * 7919 * batch) % (n - batch + 1);
-      hipLaunchKernelGGL(k_gather_batch, dim3((batch + 7) / 8), dim3(32, 8), 0, m->stream, batch,
-                         d_perm, start, d_x, d_y, m->Xb, m->Tb);
-      forward(m, member, batch);
-      backward(m, member, batch);
-      adam(m, member, lr);
-    }
+    fill(W.stream, 1, W.loss, 0.f);
+    for (int64_t st = 0; st < steps_per_epoch; ++st, ++k)
+      train_step(m, W, member, batch, batch_start(st, ep, batch, n), d_x, d_y, d_perm,
+                 cosine_lr(lr0, lr1, k, total));
     float loss = 0.f;
-    if (hipMemcpyAsync(&loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream) !=
+    if (hipMemcpyAsync(&loss, W.loss, sizeof(float), hipMemcpyDeviceToHost, W.stream) !=
             hipSuccess ||
-        hipStreamSynchronize(m->stream) != hipSuccess)
+        hipStreamSynchronize(W.stream) != hipSuccess)
       return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train: loss readback");
     if (h_epoch_loss) h_epoch_loss[ep] = loss / float(steps_per_epoch * batch * kOut);
   }
-  hipError_t e = hipStreamSynchronize(m->stream);
+  hipError_t e = hipStreamSynchronize(W.stream);
   return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train");
+}
+
+int bnsamd_mlp_train_ensemble(bnsamd_mlp* m, int64_t n, const float* d_x, const float* d_y,
+                              const int32_t* d_perm, int epochs, int batch, float lr0, float lr1,
+                              float* h_epoch_loss) {
+  if (!m || n < batch || batch < 64 || batch > m->maxb || !d_x || !d_y || !d_perm || epochs < 1)
+    return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  if (sync_all(m) != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train_ensemble: sync");
+  // one workspace and stream per member (default priority)
+  while (int(m->ws.size()) < m->members) {
+    Work W;
+    bool ok = hipStreamCreateWithFlags(&W.stream, hipStreamNonBlocking) == hipSuccess;
+    W.own_stream = ok;
+    ok = ok && work_alloc(W, m->maxb);
+    if (!ok) {
+      work_free(W);
+      return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_train_ensemble: workspace");
+    }
+    m->ws.push_back(W);
+  }
+  m->packed_valid = false;
+  const int M = m->members;
+  const int64_t steps_per_epoch = n / batch, total = steps_per_epoch * epochs;
+  int64_t k = 0;
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (int e = 0; e < M; ++e) fill(m->ws[e].stream, 1, m->ws[e].loss, 0.f);
+    // the members' steps interleaved, each on its own stream: the small
+    // GEMMs of one member leave most of the chip idle
+    for (int64_t st = 0; st < steps_per_epoch; ++st, ++k) {
+      const float lr = cosine_lr(lr0, lr1, k, total);
+      const int64_t start = batch_start(st, ep, batch, n);
+      for (int e = 0; e < M; ++e)
+        train_step(m, m->ws[e], e, batch, start, d_x, d_y, d_perm + int64_t(e) * n, lr);
+    }
+    for (int e = 0; e < M; ++e) {
+      float loss = 0.f;
+      if (hipMemcpyAsync(&loss, m->ws[e].loss, sizeof(float), hipMemcpyDeviceToHost,
+                         m->ws[e].stream) != hipSuccess ||
+          hipStreamSynchronize(m->ws[e].stream) != hipSuccess)
+        return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train_ensemble: loss readback");
+      if (h_epoch_loss)
+        h_epoch_loss[int64_t(e) * epochs + ep] = loss / float(steps_per_epoch * batch * kOut);
+    }
+  }
+  hipError_t e = sync_all(m);
+  return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_train_ensemble");
 }
 
 int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, float* d_y) {
@@ -778,16 +889,18 @@ int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, fl
   if (hipMalloc(&perm, sizeof(int32_t) * rows) != hipSuccess)
     return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_predict: hipMalloc");
   hipError_t e = hipMemcpy(perm, ident.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice);
+  Work& W = m->ws[0];
+  if (e == hipSuccess) e = sync_all(m);  // no member is still training
   for (int r0 = 0; r0 < rows && e == hipSuccess; r0 += m->maxb) {  // batches of maxb rows
     const int b = rows - r0 < m->maxb ? rows - r0 : m->maxb;
-    hipLaunchKernelGGL(k_gather_batch, dim3((b + 7) / 8), dim3(32, 8), 0, m->stream, b, perm,
-                       int64_t(r0), d_x, static_cast<const float*>(nullptr), m->Xb, m->Tb);
-    forward(m, member, b);
-    e = hipMemcpy2DAsync(d_y + int64_t(r0) * kOut, sizeof(float) * kOut, m->Z[kLayers - 1],
+    hipLaunchKernelGGL(k_gather_batch, dim3((b + 7) / 8), dim3(32, 8), 0, W.stream, b, perm,
+                       int64_t(r0), d_x, static_cast<const float*>(nullptr), W.Xb, W.Tb);
+    forward(m, W, member, b);
+    e = hipMemcpy2DAsync(d_y + int64_t(r0) * kOut, sizeof(float) * kOut, W.Z[kLayers - 1],
                          sizeof(float) * kNL, sizeof(float) * kOut, b, hipMemcpyDeviceToDevice,
-                         m->stream);
+                         W.stream);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(W.stream);
   (void)hipFree(perm);
   return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_predict");
 }
@@ -806,8 +919,8 @@ int bnsamd_surrogate_m1_opacities(bnsamd_context* ctx, bnsamd_mlp* m, int64_t np
   }
   (void)hipSetDevice(m->device);
   hipStream_t s = static_cast<hipStream_t>(bnsamd_get_stream(ctx));
-  // training runs on the MLP's own stream: parameters are final before use
-  if (hipStreamSynchronize(m->stream) != hipSuccess)
+  // training runs on the MLP's own streams: parameters are final before use
+  if (sync_all(m) != hipSuccess)
     return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate_m1_opacities: training stream");
   if (!m->packed_valid) {
     for (int e = 0; e < m->members; ++e)

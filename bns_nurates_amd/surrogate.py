@@ -171,6 +171,17 @@ class Ensemble:
                                               float(lr1), hist.ctypes.data), "bnsamd_mlp_train")
         return hist
 
+    def train_all(self, x, y, perms, epochs: int, batch: int, lr0: float, lr1: float):
+        """Every member at once (bnsamd_mlp_train_ensemble): member e reads
+        rows in the order perms[e] (device int32 [members, n]); returns the
+        epoch losses [members, epochs]."""
+        assert perms.shape == (self.members, x.shape[0]) and perms.is_contiguous()
+        hist = np.zeros((self.members, epochs), np.float32)
+        self._check(self.lib.bnsamd_mlp_train_ensemble(
+            self.h, x.shape[0], _fptr(x), _fptr(y), _fptr(perms), int(epochs), int(batch),
+            float(lr0), float(lr1), hist.ctypes.data), "bnsamd_mlp_train_ensemble")
+        return hist
+
     def predict(self, member: int, x, y):
         self._check(self.lib.bnsamd_mlp_predict(self.h, member, x.shape[0], _fptr(x), _fptr(y)),
                     "bnsamd_mlp_predict")
@@ -227,12 +238,16 @@ def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs:
     ens.set_norm(fm, fs, tm, ts, f.min(0), f.max(0), A, ys.min(0), ys.max(0))
     ens.lin = A
     ens.n_train, ens.history = int(ok.sum()), []
+    # each member from its own initialisation and data order, all trained at
+    # once on their own streams
+    perms = np.empty((members, len(f)), np.int32)
     for e in range(members):
         ens.set_params(e, init_params(seed + 17 * e))
-        perm = torch.from_numpy(np.random.default_rng(seed + 31 * e).permutation(len(f))
-                                .astype(np.int32)).cuda()
-        h = ens.train(e, x, y, perm, epochs, batch, lr0, lr1)
-        ens.history.append(h)
+        perms[e] = np.random.default_rng(seed + 31 * e).permutation(len(f))
+    hist = ens.train_all(x, y, torch.from_numpy(perms).cuda(), epochs, batch, lr0, lr1)
+    del perms
+    for e in range(members):
+        ens.history.append(hist[e])
         if log:
-            log(f"surrogate member {e}: epoch losses {np.array2string(h, precision=5)}")
+            log(f"surrogate member {e}: epoch losses {np.array2string(hist[e], precision=5)}")
     return ens

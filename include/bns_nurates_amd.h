@@ -278,6 +278,15 @@ int bnsamd_mlp_gradients(bnsamd_mlp* m, int member, int rows, const float* d_x, 
 int bnsamd_mlp_train(bnsamd_mlp* m, int member, int64_t n, const float* d_x, const float* d_y,
                      const int32_t* d_perm, int epochs, int batch, float lr0, float lr1,
                      float* h_epoch_loss);
+/* the same training for every member at once, member e on its own stream and
+ * workspace (created on first use), reading its rows in the order
+ * d_perm[e * n .. e * n + n); the members' steps are interleaved so that
+ * their small matrix products share the device.  Member e's result is the
+ * one bnsamd_mlp_train gives with d_perm + e * n; epoch losses go to
+ * h_epoch_loss[members][epochs] (may be NULL) */
+int bnsamd_mlp_train_ensemble(bnsamd_mlp* m, int64_t n, const float* d_x, const float* d_y,
+                              const int32_t* d_perm, int epochs, int batch, float lr0, float lr1,
+                              float* h_epoch_loss);
 /* standardised outputs d_y [rows][15] of one member (training-path kernels) */
 int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, float* d_y);
 /* The learned form of bnsamd_m1_opacities (same point and output blocks, on

@@ -87,3 +87,33 @@ def test_training_lowers_the_loss(ens):
                      batch=1024, lr0=2e-3, lr1=2e-4)
     say(f"training loss per epoch: {np.array2string(hist, precision=4)}")
     assert np.isfinite(hist).all() and hist[-1] < 0.2 * hist[0] and hist[-1] < 0.05
+
+
+def test_ensemble_training_matches_single_member_training(ens):
+    """bnsamd_mlp_train_ensemble trains every member at once, each on its own
+    stream and workspace; member e must end where bnsamd_mlp_train with the
+    same initialisation and row order ends (the split-K weight gradients add
+    their chunks atomically, in either order: agreement to float rounding,
+    not bit for bit)."""
+    rng = np.random.default_rng(5)
+    n = 8192
+    x = torch.from_numpy(rng.normal(size=(n, M.FEAT)).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.normal(size=(n, M.OUT)).astype(np.float32)).cuda()
+    inits = [_params(40 + e) for e in range(ens.members)]
+    perms = np.stack([rng.permutation(n).astype(np.int32) for _ in range(ens.members)])
+    for e in range(ens.members):
+        ens.set_params(e, inits[e])
+    hist = ens.train_all(x, y, torch.from_numpy(perms).cuda(), epochs=2, batch=1024, lr0=1e-3,
+                         lr1=1e-4)
+    together = [ens.get_params(e) for e in range(ens.members)]
+    for e in range(ens.members):
+        ens.set_params(e, inits[e])
+        h1 = ens.train(e, x, y, torch.from_numpy(perms[e]).cuda(), epochs=2, batch=1024,
+                       lr0=1e-3, lr1=1e-4)
+        alone = ens.get_params(e)
+        moved = np.abs(alone - inits[e]).max()
+        err = np.abs(together[e] - alone).max()
+        say(f"member {e}: losses together {hist[e]} alone {h1}; max |d w| {err:.2e} "
+            f"of a step of {moved:.2e}")
+        assert np.allclose(hist[e], h1, rtol=1e-3)
+        assert err <= 1e-2 * moved
