@@ -491,7 +491,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned form")
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral leg")
     ap.add_argument("--surrogate-train-points", type=int, default=10_000_000)
-    ap.add_argument("--surrogate-epochs", type=int, default=24)
+    ap.add_argument("--surrogate-epochs", type=int, default=48)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
