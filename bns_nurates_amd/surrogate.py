@@ -206,7 +206,7 @@ class Ensemble:
 
 
 def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs: int = 8,
-                   batch: int = 4096, lr0: float = 2e-3, lr1: float = 2e-5, seed: int = 1000,
+                   batch: int = 4096, lr0: float = 4e-3, lr1: float = 2e-5, seed: int = 1000,
                    chunk: int = 1 << 21, log=None) -> "Ensemble":
     """An ensemble trained on npts seeded points (workload distributions,
     seed `seed`) with targets from the FP64 path at quadrature order nquad;
