@@ -34,16 +34,18 @@ namespace bns {
 namespace {
 
 // Offsets (in doubles) of the dynamic LDS block of one workgroup:
+//   quadrature   x_i, w_i, 1 / x_i (correctly rounded), log x_i (i < n)
 //   node block   (pair/brem partner nodes a < N = 2n)
 //     z = nubar / T, nubar, F_1..F_5(eta - z), F_1..F_5(eta + z),
-//     TotalNuF(nubar, s), blocking factor of species s
+//     TotalNuF(nubar, s) (the blocking factor 1 - f is formed from it)
 //   energy block (e < E)
 //     nu, y = nu / T, F_1..F_5(eta - y), F_1..F_5(eta + y), F_1..F_5(-eta - y),
 //     TotalNuF(nu, s), beta em/ab (nue, anue), iso, the 8 pair/brem integrals,
 //     the 8 NEPS integrals
 //   cell block   8 integrand values of the 64 lanes of a round
 struct SpLds {
-  int z, nub, fzm, fzp, gz, bz;
+  int qx, qw, qrx, qlx;
+  int z, nub, fzm, fzp, gz;
   int nu, lnu, y, fym, fyp, fyq, gnu, bem, bab, iso, ip, in;
   int sf;
   int total;
@@ -53,12 +55,15 @@ __host__ __device__ inline SpLds sp_lds(int n, int E) {
   const int N = 2 * n;
   SpLds L;
   int o = 0;
+  L.qx = o;  o += n;
+  L.qw = o;  o += n;
+  L.qrx = o; o += n;
+  L.qlx = o; o += n;
   L.z = o;   o += N;
   L.nub = o; o += N;
   L.fzm = o; o += 5 * N;
   L.fzp = o; o += 5 * N;
   L.gz = o;  o += 4 * N;
-  L.bz = o;  o += 4 * N;
   L.nu = o;  o += E;
   L.lnu = o; o += E;
   L.y = o;   o += E;
@@ -113,19 +118,22 @@ __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
   return 3. * c::BremConst * tot * hp.medium;
 }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_spectral_f64(
-    SpectralKernelArgs A) {
+// WAVES: waves per SIMD the registers are budgeted for; NQ, ECH > 0: the
+// quadrature order and energies per chunk fixed at compile time (the LDS
+// offsets become constants instead of values held in registers), 0: taken
+// from the arguments.
+template <int WAVES, int NQ, int ECH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void
+k_spectral_f64(SpectralKernelArgs A) {
   extern __shared__ double sm[];
   __shared__ double s_in[64];  // the 21 inputs (+ 28 distribution values), 1 / T
   __shared__ Spec s_D[4];
   __shared__ double s_Fe[2][6], s_fd0[2];
   __shared__ HrChan s_hc[3];
-  __shared__ double s_x[kMaxQuad], s_w[kMaxQuad];
-  __shared__ double s_rx[kMaxQuad];  // 1 / x_i correctly rounded
-  __shared__ double s_lx[kMaxQuad];  // log(x_i)
+  __shared__ double s_brem[6];  // BRT06 prefactor; HR98 y, e^-y/12, p, -1/p, medium
 
   const int lane = threadIdx.x;
-  const int n = A.n, N = 2 * n, E = A.ech;
+  const int n = NQ > 0 ? NQ : A.n, N = 2 * n, E = ECH > 0 ? ECH : A.ech;
   // workgroup -> (point, group of cpw energy chunks); consecutive points of
   // one group share an XCD (and its L2), as in the gray kernel
   const int64_t b = xcd_point(blockIdx.x, gridDim.x);
@@ -137,6 +145,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   const volatile double* vin = s_in;
   // the LDS block: written and read in phases separated by barriers
   double* V = sm;
+  double* s_x = V + L.qx;
+  double* s_w = V + L.qw;
+  double* s_rx = V + L.qrx;  // 1 / x_i correctly rounded
+  double* s_lx = V + L.qlx;  // log(x_i)
 
   // ---- inputs -----------------------------------------------------------------
   if (lane < 21) s_in[lane] = A.pts[lane * ld + p];
@@ -192,6 +204,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     const HrChan hc = hr98_channel(lane < 3 ? lane : 0, T, vin[0], vin[2], vin[3], hp.y);
     if (lane < 3) s_hc[lane] = hc;
   }
+  // the bremsstrahlung point constants stay in LDS and are read in the cell
+  // rounds where used (held in registers through the chunk loop they spill)
+  if (lane == 0) {
+    s_brem[0] = brt_pre;
+    s_brem[1] = hp.y;
+    s_brem[2] = hp.emy;
+    s_brem[3] = hp.p;
+    s_brem[4] = hp.pinv;
+    s_brem[5] = hp.medium;
+  }
+  const volatile double* vbrem = s_brem;
   __syncthreads();
   const volatile Spec* vD = s_D;
   auto Dget = [&](int s) {
@@ -240,7 +263,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         for (int s = 0; s < 4; ++s) {
           const double g = nuf(nub, lw, Dget(s));
           V[L.gz + s * N + a] = g;
-          V[L.bz + s * N + a] = blocking ? 1. - g : 1.;
         }
       }
     }
@@ -416,11 +438,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
               // BremKernelsBRT06 at x = (w + w') / (2T) (kernel_brem_BRT06.hpp:42-69)
               const double xb = over_T(0.5 * (nu + nub));
               const double k1x = bessk1_fit(xb) / xb;
-              se = brt_pre * k1x * safe_exp(-xb);
-              sa = brt_pre * k1x * safe_exp(xb);
+              const double pre = vbrem[0];
+              se = pre * k1x * safe_exp(-xb);
+              sa = pre * k1x * safe_exp(xb);
             } else {
               const double x = over_T(nu + nub);
-              sa = hr98_abs(x, hp, s_hc);
+              const HrPoint hpl{vbrem[1], vbrem[2], vbrem[3], vbrem[4], vbrem[5]};
+              sa = hr98_abs(x, hpl, s_hc);
               se = sa * safe_exp(-x);
             }
           }
@@ -431,7 +455,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   #pragma unroll
           for (int s = 0; s < 4; ++s) {
             const int ps = s ^ 1;
-            const double pro = (em[s] + se) * V[L.bz + ps * N + a];
+            const double pro = (em[s] + se) * (blocking ? 1. - V[L.gz + ps * N + a] : 1.);
             const double ann = (ab[s] + sa) * V[L.gz + ps * N + a];
             V[L.sf + s * kWave + lane] = (nb2 * pro) * jac;
             V[L.sf + (4 + s) * kWave + lane] = (nb2 * ann) * jac;
@@ -587,7 +611,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 
 }  // namespace
 
+// The instance for the reference's default order, n = 10: 6 energies per
+// chunk keep the workgroup's LDS (1 KB static + 9 KB) within the 10 KB that
+// 4 waves per SIMD allow, and the compile-time layout keeps it within 128
+// VGPRs without spills.  Other orders: 3 waves, up to four rounds per chunk.
+constexpr int kFixedN = 10, kFixedE = 6;
+
 int spectral_energies_per_chunk(int n) {
+  if (n == kFixedN) return kFixedE;
   const int G = kWave / (2 * n);
   return std::min(4 * G, 16);
 }
@@ -611,7 +642,11 @@ hipError_t launch_spectral_f64(const SpectralKernelArgs& a, hipStream_t stream) 
   if (a.npts <= 0 || a.nnu <= 0) return hipSuccess;
   const int64_t groups = a.npts * ((a.nchunk + a.cpw - 1) / a.cpw);
   dim3 grid(static_cast<unsigned>(groups)), block(kWave);
-  hipLaunchKernelGGL(k_spectral_f64, grid, block, spectral_lds_bytes(a.n, a.ech), stream, a);
+  const size_t lds = spectral_lds_bytes(a.n, a.ech);
+  if (a.n == kFixedN && a.ech == kFixedE)
+    hipLaunchKernelGGL((k_spectral_f64<4, kFixedN, kFixedE>), grid, block, lds, stream, a);
+  else
+    hipLaunchKernelGGL((k_spectral_f64<3, 0, 0>), grid, block, lds, stream, a);
   return hipGetLastError();
 }
 
