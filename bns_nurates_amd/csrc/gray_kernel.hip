@@ -32,7 +32,9 @@ constexpr int kEtaSlot = 61;
 // functions, everything else FP64).  STIM (stimulated absorption,
 // A.stim == 1) and BLOCK (partner blocking, neglect_blocking == 0) are
 // fixed per launch, so that a cell evaluates only its own weighting.
-template <bool MIX, bool STIM, bool BLOCK>
+// NQ > 0: both quadrature orders fixed at NQ at compile time (table
+// offsets and loop bounds are constants); 0: taken from the arguments.
+template <bool MIX, bool STIM, bool BLOCK, int NQ>
 __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
   __shared__ double s_Fp[5][64];  // F_1..F_5 (eta + y_a)
@@ -58,7 +60,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   if (p >= A.npts) return;
   const int64_t ld = A.ld;
   const double* __restrict__ tb = A.tab;
-  const TableLayout L = A.L;
+  const TableLayout L = NQ > 0 ? gray_table_layout(NQ, NQ) : A.L;
   const volatile double* vin = s_in;
 
   // ---- point inputs: lane f loads field f ----------------------------------
@@ -705,18 +707,18 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   if (lane < 20) A.out[lane * ld + p] = vin[lane];
 }
 
-template <int WAVES_PER_EU, bool STIM, bool BLOCK>
+template <int WAVES_PER_EU, bool STIM, bool BLOCK, int NQ>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_f64(
     GrayKernelArgs A) {
-  gray_point<false, STIM, BLOCK>(A);
+  gray_point<false, STIM, BLOCK, NQ>(A);
 }
 
-template <int WAVES_PER_EU, bool STIM, bool BLOCK>
+template <int WAVES_PER_EU, bool STIM, bool BLOCK, int NQ>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_mixed(
     GrayKernelArgs A) {
-  gray_point<true, STIM, BLOCK>(A);
+  gray_point<true, STIM, BLOCK, NQ>(A);
 }
 
 }  // namespace bns
@@ -729,10 +731,20 @@ void launch_instance(const GrayKernelArgs& a, dim3 grid, hipStream_t stream) {
   // per cell round); the mixed form with blocking keeps its 2 KB blocking
   // table, and its LDS allows 3.7 workgroups per SIMD, so it stays at 3
   constexpr int W = (MIX && BLOCK) ? 3 : 4;
-  if constexpr (MIX)
-    hipLaunchKernelGGL((k_gray_m1_mixed<W, STIM, BLOCK>), grid, dim3(64), 0, stream, a);
-  else
-    hipLaunchKernelGGL((k_gray_m1_f64<W, STIM, BLOCK>), grid, dim3(64), 0, stream, a);
+  // the 128^3-grid workload's order, n = 16, has its own instance (0.6%
+  // faster: its table offsets and loop bounds are constants)
+  const bool fixed = a.L.n1 == 16 && a.L.n2 == 16;
+  if constexpr (MIX) {
+    if (fixed)
+      hipLaunchKernelGGL((k_gray_m1_mixed<W, STIM, BLOCK, 16>), grid, dim3(64), 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_gray_m1_mixed<W, STIM, BLOCK, 0>), grid, dim3(64), 0, stream, a);
+  } else {
+    if (fixed)
+      hipLaunchKernelGGL((k_gray_m1_f64<W, STIM, BLOCK, 16>), grid, dim3(64), 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_gray_m1_f64<W, STIM, BLOCK, 0>), grid, dim3(64), 0, stream, a);
+  }
 }
 // the instance for the launch's stimulated-absorption and blocking switches
 template <bool MIX>

@@ -11,21 +11,6 @@
 
 namespace bns {
 
-TableLayout gray_table_layout(int n1, int n2) {
-  TableLayout L{};
-  L.n1 = n1;
-  L.n2 = n2;
-  L.N = 2 * n2;
-  L.P = L.N * (L.N + 1) / 2;
-  L.Q = 2 * n2 * n2;
-  L.q1 = 0;
-  L.nd = L.q1 + Q1_REC * n1 + (n1 & 1) * Q1_REC;
-  L.cl = L.nd + ND_REC * L.N;
-  L.np = L.cl + CL_REC * L.P;
-  L.total = L.np + NP_REC * L.Q;
-  return L;
-}
-
 TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2,
                               const double* x2, const double* w2, double* block) {
   TableLayout L = gray_table_layout(n1, n2);

@@ -66,7 +66,26 @@ struct TableLayout {
   int total;           // doubles in the block
 };
 
-TableLayout gray_table_layout(int n1, int n2);
+// record counts and offsets for quadrature orders n1 (1D) and n2 (2D); a
+// constant expression, so that a kernel instance for fixed orders gets its
+// offsets and loop bounds at compile time
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+constexpr TableLayout gray_table_layout(int n1, int n2) {
+  TableLayout L{};
+  L.n1 = n1;
+  L.n2 = n2;
+  L.N = 2 * n2;
+  L.P = L.N * (L.N + 1) / 2;
+  L.Q = 2 * n2 * n2;
+  L.q1 = 0;
+  L.nd = L.q1 + Q1_REC * n1 + (n1 & 1) * Q1_REC;
+  L.cl = L.nd + ND_REC * L.N;
+  L.np = L.cl + CL_REC * L.P;
+  L.total = L.np + NP_REC * L.Q;
+  return L;
+}
 
 // Fills `block` (layout.total doubles) for the quadratures (x1, w1: n1 nodes)
 // and (x2, w2: n2 nodes) on [0, 1].
