@@ -256,8 +256,9 @@ __global__ void k_act_bwd(int rows, int cols, const float* __restrict__ dH,
   atomicAdd(&db[c], s);
 }
 
-// last layer: y = Z (bias added), e = y - t on the first nout columns;
-// loss += sum e^2, dZ = 2 e / (rows nout) as BF16, db += column sums
+// last layer: y = Z (bias added), e = y - t on the first nout columns (0
+// where the target is NaN: an output missing for that row); loss += sum e^2,
+// dZ = 2 e / (rows nout) as BF16, db += column sums
 __global__ void k_mse_grad(int rows, int cols, int nout, const float* __restrict__ Z,
                            const float* __restrict__ T, uint16_t* __restrict__ dZ,
                            float* __restrict__ db, float* __restrict__ loss) {
@@ -269,7 +270,8 @@ __global__ void k_mse_grad(int rows, int cols, int nout, const float* __restrict
   if (rl < rstep) {
     for (int r = r0 + rl; r < min(rows, r0 + 256); r += rstep) {
       int64_t i = int64_t(r) * cols + c;
-      float e = c < nout ? Z[i] - T[i] : 0.f;
+      const float tv = T[i];
+      float e = (c < nout && tv == tv) ? Z[i] - tv : 0.f;
       float d = e * scale;
       dZ[i] = f2bf(d);
       s += d;
@@ -387,6 +389,18 @@ __device__ inline void mma_layer_t(const uint16_t* __restrict__ A, int stride,
   }
 }
 
+// Whether the FP64 path's density thresholds set output kind q (0 eta_0,
+// 1 eta, 2 kappa_0_a, 3 kappa_a, 4 kappa_s) of species sp to zero
+// (m1_opacities.hpp:1199-1292): kappa_0_a needs n > 1e-21, kappa_a and
+// kappa_s need J > 1e-25, and kappa_a of nue is zero when n is.  Such
+// outputs are not the network's (training masks them), so they neither
+// enter the spread test nor take its value.
+__device__ inline bool threshold_zero(int q, int sp, double ns, double Js) {
+  if (q == 2) return !(ns > bns::c::ThresholdN);
+  if (q >= 3) return !(Js > bns::c::ThresholdJ) || (q == 3 && sp == 0 && ns == 0.);
+  return false;
+}
+
 // bias + SiLU of one transposed tile, four features of one point per lane,
 // written as one 8-byte row segment of H
 __device__ inline void store_act4(uint16_t* __restrict__ dst, floatx4 v, const float4 b) {
@@ -501,6 +515,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
   // ensemble mean -> outputs; spread (natural-log units) -> fallback flag
   const int r = 16 * wave + (lane & 15);
+  const int64_t p = p0 + r;
+  const bool live = p < a.npts;
   const float inv = 1.f / float(a.members);
   float mean[4];
 #pragma unroll
@@ -510,9 +526,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const float rlo = c < kOut ? nm.res_lo[c] : 0.f, rhi = c < kOut ? nm.res_hi[c] : 0.f;
     mean[v] = sum[v] * inv;
     float var = fmaxf(sq[v] * inv - mean[v] * mean[v], 0.f);
+    bool zero = false;
+    if (c < kOut && live) {
+      const int sp = c / 5;
+      zero = threshold_zero(c % 5, sp, a.pts[(9 + sp) * a.ld + p], a.pts[(13 + sp) * a.ld + p]);
+    }
     // a residual outside the training range counts as unbounded spread
     bool in_range = mean[v] >= rlo && mean[v] <= rhi;
-    spread[r][c] = c < kOut ? (in_range ? sqrtf(var) * os : __builtin_huge_valf()) : 0.f;
+    spread[r][c] = (c < kOut && !zero) ? (in_range ? sqrtf(var) * os : __builtin_huge_valf())
+                                       : 0.f;
   }
   __syncthreads();
   if (t < kRows) {
@@ -531,14 +553,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // outputs of the accepted points: column c = 5 s + q (q: eta_0, eta,
   // kappa_0_a, kappa_a, kappa_s) goes to field 4 q' + s, anux copying nux.
   // The opacities keep the zeros the FP64 path gives below the density
-  // thresholds (m1_opacities.hpp:1199-1292): kappa_0_a needs n > 1e-21,
-  // kappa_a and kappa_s need J > 1e-25, and kappa_a of nue is zero when n is
-  // (for anux, its own n and J decide).
-  const int64_t p = p0 + r;
+  // thresholds (for anux, its own n and J decide).
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int c = q4 + v;
-    if (c < kOut && p < a.npts && !oob[r]) {
+    if (c < kOut && live && !oob[r]) {
       const int s = c / 5, q = c % 5;
       const int base = q == 0 ? 0 : q == 1 ? 8 : q == 2 ? 4 : q == 3 ? 12 : 16;
       float lin = nm.lin[0][c];
@@ -547,9 +566,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const double y = exp(double(lin) + double(mean[v] * nm.out_std[c] + nm.out_mean[c]));
       auto value = [&](int sp) {
         const double ns = a.pts[(9 + sp) * a.ld + p], Js = a.pts[(13 + sp) * a.ld + p];
-        if (q == 2) return ns > bns::c::ThresholdN ? y : 0.;
-        if (q >= 3) return (Js > bns::c::ThresholdJ && !(q == 3 && sp == 0 && ns == 0.)) ? y : 0.;
-        return y;
+        return threshold_zero(q, sp, ns, Js) ? 0. : y;
       };
       a.out[(base + s) * a.ld + p] = value(s);
       if (s == 2) a.out[(base + 3) * a.ld + p] = value(3);

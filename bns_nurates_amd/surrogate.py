@@ -53,12 +53,33 @@ def features(pts: np.ndarray) -> np.ndarray:
 
 
 def targets(out: np.ndarray):
-    """Log outputs [npts, 15] of an output block [20, npts] and the mask of
-    points whose 15 outputs are all positive and finite."""
+    """Log outputs [npts, 15] of an output block [20, npts], NaN where an
+    output is not positive and finite (the FP64 path's threshold zeros: such
+    a target is missing, not trained on), and the mask of points with at
+    least one target."""
     o = out[OUT_ROWS].T
-    ok = np.all(np.isfinite(o) & (o > 0), axis=1)
+    pos = np.isfinite(o) & (o > 0)
     with np.errstate(divide="ignore", invalid="ignore"):
-        return np.log(np.where(o > 0, o, 1.0)), ok
+        t = np.where(pos, np.log(np.where(pos, o, 1.0)), np.nan)
+    return t, np.any(pos, axis=1)
+
+
+def masked_linear_fit(X: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """Least-squares coefficients [cols, outputs] of t ~ X per output, each
+    on the rows where its target is present (outputs sharing a mask of
+    present rows are fitted together)."""
+    pres = ~np.isnan(t)
+    A = np.zeros((X.shape[1], t.shape[1]))
+    keys = [pres[:, o].tobytes() for o in range(t.shape[1])]
+    done = set()
+    for o in range(t.shape[1]):
+        if o in done:
+            continue
+        group = [j for j in range(t.shape[1]) if keys[j] == keys[o]]
+        rows = pres[:, o]
+        A[:, group] = np.linalg.lstsq(X[rows], t[rows][:, group], rcond=None)[0]
+        done.update(group)
+    return A
 
 
 # layer shapes (padded) and the parameter blob layout of bns_nurates_amd/csrc/mlp.h
@@ -207,7 +228,8 @@ class Ensemble:
 
 def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs: int = 8,
                    batch: int = 4096, lr0: float = 4e-3, lr1: float = 2e-5, seed: int = 1000,
-                   chunk: int = 1 << 21, log=None) -> "Ensemble":
+                   chunk: int = 1 << 21, partial_targets: bool = False,
+                   log=None) -> "Ensemble":
     """An ensemble trained on npts seeded points (workload distributions,
     seed `seed`) with targets from the FP64 path at quadrature order nquad;
     member e starts from its own initialisation and data order."""
@@ -221,21 +243,33 @@ def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs:
     out = np.concatenate(outs, axis=1)
     with np.errstate(divide="ignore"):
         f = features(pts)
+    # points with finite features and all 15 outputs positive; with
+    # partial_targets also those where the FP64 path sets some outputs to
+    # zero (n or J below its thresholds): those outputs are missing targets,
+    # left out of the fits and the loss.  (Measured on the 128^3 grid: the
+    # training box then holds 99.99% of the points instead of 93.2%, but the
+    # extreme-degeneracy points it adds cost accuracy elsewhere -- all 15
+    # outputs within 3e-2 on 12.8% of the points instead of 31.8% -- for the
+    # same speed, so they are left out by default.)
     t, ok = targets(out)
+    if not partial_targets:
+        ok &= ~np.any(np.isnan(t), axis=1)
     ok &= np.all(np.isfinite(f), axis=1)
     f, t = f[ok], t[ok]
     fm, fs = f.mean(0), f.std(0)
     xn = (f - fm) / fs
     # linear model of the log outputs in the standardised features; the
     # network learns the standardised residual
-    A, *_ = np.linalg.lstsq(np.c_[np.ones(len(xn)), xn], t, rcond=None)
-    res = t - np.c_[np.ones(len(xn)), xn] @ A
-    tm, ts = res.mean(0), res.std(0)
-    x = torch.from_numpy(xn.astype(np.float32)).cuda()
-    y = torch.from_numpy(((res - tm) / ts).astype(np.float32)).cuda()
-    ens = Ensemble(ctx.device, members=members, max_batch=batch)
+    X1 = np.c_[np.ones(len(xn)), xn]
+    A = masked_linear_fit(X1, t)
+    res = t - X1 @ A
+    del X1
+    tm, ts = np.nanmean(res, 0), np.nanstd(res, 0)
     ys = (res - tm) / ts
-    ens.set_norm(fm, fs, tm, ts, f.min(0), f.max(0), A, ys.min(0), ys.max(0))
+    x = torch.from_numpy(xn.astype(np.float32)).cuda()
+    y = torch.from_numpy(ys.astype(np.float32)).cuda()
+    ens = Ensemble(ctx.device, members=members, max_batch=batch)
+    ens.set_norm(fm, fs, tm, ts, f.min(0), f.max(0), A, np.nanmin(ys, 0), np.nanmax(ys, 0))
     ens.lin = A
     ens.n_train, ens.history = int(ok.sum()), []
     # each member from its own initialisation and data order, all trained at

@@ -69,7 +69,9 @@ def forward(params, x, exact=False):
 
 def gradients(params, x, y, exact=False):
     """Loss and gradient blob of the mean-squared error over the 15 outputs
-    (rounded as the kernels round, or exact as in forward)."""
+    (rounded as the kernels round, or exact as in forward).  A NaN target is
+    a missing one (an output the FP64 path sets to zero): its error counts as
+    0; the normalisation stays rows * 15."""
     rnd = (lambda a: np.asarray(a, np.float64)) if exact else bf16
     dt = np.float64 if exact else np.float32
     W, _ = unpack(params)
@@ -81,6 +83,7 @@ def gradients(params, x, y, exact=False):
     t[:, :OUT] = y
     e = zs[-1].astype(np.float64) - t
     e[:, OUT:] = 0.0
+    e[np.isnan(e)] = 0.0
     loss = float((e ** 2).sum() / (rows * OUT))
     d = (e * (2.0 / (rows * OUT))).astype(dt)
     for l in range(LAYERS - 1, -1, -1):

@@ -46,12 +46,17 @@ def test_forward_matches_oracle(ens):
     assert err < 2e-2
 
 
-def test_gradients_match_oracle(ens):
+@pytest.mark.parametrize("missing", [0.0, 0.15])
+def test_gradients_match_oracle(ens, missing):
+    """Loss and gradients on the matrix cores against the oracle; with
+    `missing` of the targets NaN (outputs the FP64 path sets to zero), which
+    both leave out of the loss."""
     rng = np.random.default_rng(4)
     p = _params(5)
     ens.set_params(1, p)
     x = rng.normal(size=(512, M.FEAT)).astype(np.float32)
     y = rng.normal(size=(512, M.OUT)).astype(np.float32)
+    y[rng.random(y.shape) < missing] = np.nan
     loss, g = ens.gradients(1, torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
     lo, go = M.gradients(p, x, y)
     wo, bo, _ = M.offsets()
@@ -78,9 +83,10 @@ def test_training_lowers_the_loss(ens):
     ctx.close()
     f = S.features(pts)
     t, ok = S.targets(out)
+    ok &= np.all(np.isfinite(f), axis=1)
     f, t = f[ok], t[ok]
     x = ((f - f.mean(0)) / f.std(0)).astype(np.float32)
-    y = ((t - t.mean(0)) / t.std(0)).astype(np.float32)
+    y = ((t - np.nanmean(t, 0)) / np.nanstd(t, 0)).astype(np.float32)
     ens.set_params(0, S.init_params(7))
     perm = torch.from_numpy(np.random.default_rng(0).permutation(len(x)).astype(np.int32)).cuda()
     hist = ens.train(0, torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), perm, epochs=6,

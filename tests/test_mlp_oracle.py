@@ -42,3 +42,30 @@ def test_gradients_are_the_derivative_of_the_loss():
         pm[i] -= h
         fd = (M.gradients(pp, x, y, exact=True)[0] - M.gradients(pm, x, y, exact=True)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_missing_targets_carry_no_error():
+    """A NaN target (an output the FP64 path sets to zero) is left out of the
+    loss: the gradients equal those with that target replaced by the
+    network's own output, and the loss keeps its rows * 15 normalisation."""
+    rng = np.random.default_rng(8)
+    p = S.init_params(2).astype(np.float64)
+    x = rng.normal(size=(32, M.FEAT))
+    y = rng.normal(size=(32, M.OUT))
+    miss = rng.random(y.shape) < 0.2
+    zs, _ = M.forward(p, x, exact=True)
+    y_nan, y_own = y.copy(), y.copy()
+    y_nan[miss] = np.nan
+    y_own[miss] = zs[-1][:, :M.OUT][miss]
+    l1, g1 = M.gradients(p, x, y_nan, exact=True)
+    l2, g2 = M.gradients(p, x, y_own, exact=True)
+    # the targets are held in float32: a target equal to the output carries
+    # the output's float32 rounding (relative 6e-8) as its error
+    assert np.isfinite(g1).all() and np.isfinite(l1)
+    assert abs(l1 - l2) <= 1e-12 * l2
+    assert np.abs(g1 - g2).max() <= 1e-6 * np.abs(g2).max()
+    # and a missing target is not a zero target
+    y_zero = y.copy()
+    y_zero[miss] = 0.0
+    _, g3 = M.gradients(p, x, y_zero, exact=True)
+    assert np.abs(g1 - g3).max() > 1e-3 * np.abs(g3).max()
