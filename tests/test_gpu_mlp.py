@@ -121,5 +121,7 @@ def test_ensemble_training_matches_single_member_training(ens):
         err = np.abs(together[e] - alone).max()
         say(f"member {e}: losses together {hist[e]} alone {h1}; max |d w| {err:.2e} "
             f"of a step of {moved:.2e}")
+        # measured: 5e-4 to 3e-3 of the step (the atomic adds' order); a
+        # member trained on another's data or stream would differ by O(1)
         assert np.allclose(hist[e], h1, rtol=1e-3)
-        assert err <= 1e-2 * moved
+        assert err <= 5e-2 * moved
