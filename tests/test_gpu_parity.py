@@ -159,6 +159,22 @@ def test_mixed_orders_and_blocking_vs_oracle(ctx):
         assert check_parity(f"n1={n1} n2={n2}", out, ref, m, W.OUT_NAMES) <= 1.0
 
 
+@pytest.mark.parametrize("brem,stim,blk,prec", [(W.BREM_BRT06, 1, 1, 0), (W.BREM_HR98, 0, 0, 0),
+                                                (W.BREM_BRT06, 1, 0, 1)])
+def test_max_order_vs_oracle(ctx, brem, stim, blk, prec):
+    """The largest quadrature the API accepts (n = 32: 64 nodes, one per
+    lane of a wavefront) against the oracle; FP64 to the FP64 bar, the mixed
+    form (precision 1) to 1e-5."""
+    pts = W.make_points(96, seed=33)
+    prm = W.params_default_all(brem)
+    prm[7] = blk
+    say(f"n=32 brem={brem} stim={stim} neglect_blocking={blk} precision={prec}")
+    ref, m = O.bound_m1_opacities(pts, 32, W.FLAGS_ALL, prm, stim=stim)
+    out = run(ctx, pts, 32, W.FLAGS_ALL, prm, stim=stim, precision=prec)
+    rtol = TOL64 if prec == 0 else 1e-5
+    assert check_parity(f"n=32 precision={prec}", out, ref, m, W.OUT_NAMES, rtol=rtol) <= 1.0
+
+
 def test_batch_invariance(ctx):
     """A point's result does not depend on the batch it is computed in."""
     pts = W.make_points(3000, seed=4)
