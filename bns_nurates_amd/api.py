@@ -205,6 +205,9 @@ def load_library():
         "bnsamd_mlp_train_ensemble": ([vp, i64, vp, vp, vp, C.c_int, C.c_int, C.c_float,
                                        C.c_float, vp], C.c_int),
         "bnsamd_mlp_predict": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
+        "bnsamd_mlp_set_domain": ([vp, vp], C.c_int),
+        "bnsamd_mlp_set_acceptance": ([vp, vp, vp], C.c_int),
+        "bnsamd_mlp_infer": ([vp, vp, i64, i64, vp, vp, vp, vp], C.c_int),
         "bnsamd_surrogate_m1_opacities": ([vp, vp, i64, i64, vp, C.c_float, C.POINTER(_Flags),
                                            C.POINTER(_Params), C.c_int, vp, C.POINTER(i64)],
                                           C.c_int),
@@ -316,12 +319,23 @@ class Context:
 
         Device tensors run asynchronously on the context stream and write
         ``out`` (allocated when None); numpy arrays go through the
-        synchronous host entry point and return a numpy array."""
+        synchronous host entry point and return a numpy array (``out``, a
+        C-contiguous float64 [20, npts] array such as a PinnedBuffer's,
+        when given)."""
         fl, pr = as_flags(flags).c(), as_params(params).c()
         if isinstance(pts, np.ndarray):
             pts = np.ascontiguousarray(pts, dtype=np.float64)
             n = pts.shape[1] if npts is None else npts
-            res = np.empty((OUT_NFIELDS, n))
+            if pts.shape[0] != PT_NFIELDS or pts.shape[1] != n:
+                raise BnsAmdError(f"pts: expected a [{PT_NFIELDS}, {n}] host array")
+            if out is None:
+                res = np.empty((OUT_NFIELDS, n))
+            elif (isinstance(out, np.ndarray) and out.dtype == np.float64 and
+                  out.flags.c_contiguous and out.shape == (OUT_NFIELDS, n)):
+                res = out
+            else:
+                raise BnsAmdError(f"out: expected a C-contiguous float64 [{OUT_NFIELDS}, {n}] "
+                                  "host array")
             dd = None
             if distr is not None:
                 distr = np.ascontiguousarray(distr, dtype=np.float64)
@@ -418,7 +432,10 @@ class Context:
         pp, ld = _tensor_ptr(pts, PT_NFIELDS, n, "pts")
         if distr is None:
             distr = torch.empty((DI_NFIELDS, ld), dtype=torch.float64, device=pts.device)
-        dp, _ = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+        dp, ldd = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+        if ldd != ld:
+            # the kernel addresses both blocks with one leading dimension
+            raise BnsAmdError("distr must have the leading dimension of pts")
         self._check(self.lib.bnsamd_distr_from_m1(self.h, n, ld, pp, dp), "bnsamd_distr_from_m1")
         return distr
 
@@ -430,7 +447,10 @@ class Context:
         pp, ld = _tensor_ptr(pts, PT_NFIELDS, n, "pts")
         if distr is None:
             distr = torch.empty((DI_NFIELDS, ld), dtype=torch.float64, device=pts.device)
-        dp, _ = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+        dp, ldd = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
+        if ldd != ld:
+            raise BnsAmdError("distr must have the leading dimension of pts")
+        self._follow_torch_stream()
         self._check(self.lib.bnsamd_equilibrium(self.h, n, ld, pp, dp), "bnsamd_equilibrium")
         return distr
 

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/bns_nurates_amd.h"
+#include "context.h"
 #include "gray_kernel.h"
 #include "gray_math.h"
 #include "gray_tables.h"
@@ -306,9 +307,14 @@ int bnsamd_set_stream(bnsamd_context* c, void* s) {
   if (check_ctx(c)) return BNSAMD_ERR_INVALID;
   std::lock_guard<std::mutex> g(c->mu);
   hipStream_t ns = static_cast<hipStream_t>(s);  // NULL: HIP's default (null) stream
-  // the staging buffer of the host entry is reused in stream order: finish
-  // what the previous stream still has queued on it
-  if (ns != c->stream && c->stage) (void)hipStreamSynchronize(c->stream);
+  // buffers the library reuses in stream order (the host entry's staging
+  // buffer, the learned form's fallback workspace) must not be in use on two
+  // streams at once: finish what the previous stream still has queued
+  if (ns != c->stream) {
+    (void)hipSetDevice(c->device);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "bnsamd_set_stream: previous stream");
+  }
   c->stream = ns;
   return BNSAMD_OK;
 }
@@ -598,3 +604,12 @@ int bnsamd_fdi(bnsamd_context* c, int k, int64_t n, const double* d_y, double* d
 }
 
 }  // extern "C"
+
+bool bns::context_quadrature_orders(bnsamd_context* c, int* n1, int* n2) {
+  if (!c) return false;
+  std::lock_guard<std::mutex> g(c->mu);
+  if (!c->current) return false;
+  *n1 = c->current->L.n1;
+  *n2 = c->current->L.n2;
+  return true;
+}

@@ -1,6 +1,6 @@
 // mlp.h -- shapes and parameter layout of the surrogate MLP (one ensemble
-// member): 15 features (padded to 32) -> 4 x 256 (SiLU) -> 15 outputs
-// (padded to 16).  Parameters of a member are one FP32 blob:
+// member): 15 features (as 30 hi/lo BF16 inputs, padded to 32) -> 4 x 256
+// (SiLU) -> 15 outputs (padded to 16).  Parameters of a member are one FP32 blob:
 // W_0, b_0, W_1, b_1, ..., W_l row-major [K_l][N_l].
 #ifndef BNS_NURATES_AMD_MLP_H_
 #define BNS_NURATES_AMD_MLP_H_
@@ -10,6 +10,11 @@ namespace mlp {
 
 constexpr int kFeat = 15;     // input features
 constexpr int kOut = 15;      // outputs
+// Each standardised feature x enters the first layer as two BF16 inputs, x_hi
+// = bf16(x) in column i and x_lo = bf16(x - x_hi) in column kFeat + i, with
+// the same weights (the rows kFeat .. 2 kFeat - 1 of W_0 mirror rows
+// 0 .. kFeat - 1 in the BF16 copies; the FP32 master rows stay zero): the
+// product W_0 x then carries ~16 significant bits of x instead of 8.
 constexpr int kK0 = 32;       // padded input width (one MFMA K step)
 constexpr int kNL = 16;       // padded output width
 constexpr int kHidden = 256;  // hidden width

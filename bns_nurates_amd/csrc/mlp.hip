@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/bns_nurates_amd.h"
+#include "context.h"
 #include "gray_math.h"
 #include "mlp.h"
 
@@ -313,7 +314,14 @@ __global__ void k_gather_batch(int B, const int32_t* __restrict__ perm, int64_t 
   int r = blockIdx.x * blockDim.y + threadIdx.y, c = threadIdx.x;  // 32 x 8
   if (r >= B) return;
   int64_t src = perm[start + r];
-  Xb[int64_t(r) * kK0 + c] = c < kFeat ? f2bf(X[src * kFeat + c]) : uint16_t(0);
+  uint16_t h = 0;
+  if (c < kFeat) {
+    h = f2bf(X[src * kFeat + c]);
+  } else if (c < 2 * kFeat) {  // low part of feature c - kFeat
+    const float x = X[src * kFeat + c - kFeat];
+    h = f2bf(x - bf2f(f2bf(x)));
+  }
+  Xb[int64_t(r) * kK0 + c] = h;
   if (Y && c < kNL) Tb[int64_t(r) * kNL + c] = c < kOut ? Y[src * kOut + c] : 0.f;
 }
 
@@ -322,16 +330,38 @@ __global__ void k_fill(int64_t n, float* __restrict__ p, float v) {
   if (i < n) p[i] = v;
 }
 
+// tied first layer: the gradient of the low-part rows adds into the rows of
+// their features (and is cleared), so that Adam moves the shared weights
+__global__ void k_tie_grad_l0(float* __restrict__ g) {
+  const int n = threadIdx.x;  // kHidden threads
+#pragma unroll
+  for (int i = 0; i < kFeat; ++i) {
+    g[i * kHidden + n + kWOff[0]] += g[(kFeat + i) * kHidden + n + kWOff[0]];
+    g[(kFeat + i) * kHidden + n + kWOff[0]] = 0.f;
+  }
+}
+// the BF16 copy of W_0: low-part rows mirror the feature rows
+__global__ void k_mirror_l0(uint16_t* __restrict__ wb) {
+  const int n = threadIdx.x;  // kHidden threads
+#pragma unroll
+  for (int i = 0; i < kFeat; ++i)
+    wb[(kFeat + i) * kHidden + n + kWOff[0]] = wb[i * kHidden + n + kWOff[0]];
+}
+
 // ---- inference ---------------------------------------------------------------------
 // BF16 weights of one layer in MFMA fragment order: block (kb, nb) of 32 x 16
 // holds, for lane l, W[32 kb + 8 (l / 16) + j][16 nb + l % 16], j = 0..7.
-__global__ void k_pack(int K, int N, const float* __restrict__ Wf, uint16_t* __restrict__ Wp) {
+// mirror: the first layer, whose rows kFeat .. 2 kFeat - 1 take the weights
+// of rows 0 .. kFeat - 1.
+__global__ void k_pack(int K, int N, const float* __restrict__ Wf, uint16_t* __restrict__ Wp,
+                       int mirror) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= int64_t(K) * N) return;
   int j = int(i & 7), lane = int((i >> 3) & 63);
   int64_t blk = i >> 9;
   int NB = N / 16, kb = int(blk / NB), nb = int(blk % NB);
   int k = 32 * kb + 8 * (lane >> 4) + j, n = 16 * nb + (lane & 15);
+  if (mirror && k >= kFeat && k < 2 * kFeat) k -= kFeat;
   Wp[i] = f2bf(Wf[int64_t(k) * N + n]);
 }
 
@@ -342,10 +372,15 @@ struct InferArgs {
   const uint16_t* wp[kMaxMembers];   // packed weights per member
   const float* w[kMaxMembers];       // FP32 blobs (biases)
   int members;
-  float threshold;                   // ensemble spread (natural log) that triggers fallback
+  float threshold;                   // predicted log error that triggers fallback
+  double dU, dm_eff;                 // training-domain values of these point fields
   double* out;                       // [20][ld]
   int32_t* fallback_idx;             // compacted indices of the points to recompute
   int32_t* fallback_count;
+  // calibration form: raw predictions instead of outputs and fallback
+  double* cal_mean;                  // [15][ld] mean log output
+  float* cal_spread;                 // [15][ld] members' spread of the log output
+  int32_t* cal_outside;              // [ld] outside the training box or domain
 };
 
 constexpr int kRows = 64, kLds = kHidden + 8;  // points per workgroup, padded row
@@ -417,7 +452,9 @@ __device__ inline void store_act4(uint16_t* __restrict__ dst, floatx4 v, const f
 // LDS holds the points' BF16 features and one activation buffer, updated in
 // place: a layer's outputs stay in the MFMA accumulators until every wave
 // has read its inputs.  212 VGPRs: 2 waves per SIMD (3 spill, and measured
-// no faster).
+// no faster).  CAL: write the raw predictions (mean log output, spread,
+// outside flag) instead of outputs and fallback indices.
+template <bool CAL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_infer(InferArgs a) {
   __shared__ __attribute__((aligned(16))) uint16_t X[kRows][kXLd];
   __shared__ __attribute__((aligned(16))) uint16_t H[kRows][kLds];
@@ -450,6 +487,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       f[12] = float(q[6 * ld] * rT);                // mu_e / T
       f[13] = float((q[5 * ld] - q[4 * ld]) * rT);  // (mu_n - mu_p) / T
       f[14] = float(rT);
+      // the inputs that are not features must take their training-domain
+      // values: anux moments equal to nux's, yn = 1 - yp, dU and dm_eff
+      in = q[12 * ld] == q[11 * ld] && q[16 * ld] == q[15 * ld] && q[20 * ld] == q[19 * ld] &&
+           q[3 * ld] == 1. - q[2 * ld] && q[7 * ld] == a.dU && q[8 * ld] == a.dm_eff;
     } else {
 #pragma unroll
       for (int i = 0; i < kFeat; ++i) f[i] = nm.feat_mean[i];
@@ -460,10 +501,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       out_of_box = out_of_box || !(f[i] >= nm.box_lo[i] && f[i] <= nm.box_hi[i]);
       const float x = (f[i] - nm.feat_mean[i]) / nm.feat_std[i];
       xs[t][i] = out_of_box ? 0.f : x;
-      X[t][i] = f2bf(x);
+      const uint16_t hi = f2bf(x);
+      X[t][i] = hi;
+      X[t][kFeat + i] = f2bf(x - bf2f(hi));
     }
 #pragma unroll
-    for (int i = kFeat; i < kXLd; ++i) X[t][i] = 0;
+    for (int i = 2 * kFeat; i < kXLd; ++i) X[t][i] = 0;
     oob[t] = out_of_box;
   }
   __syncthreads();
@@ -533,8 +576,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
     // a residual outside the training range counts as unbounded spread
     bool in_range = mean[v] >= rlo && mean[v] <= rhi;
-    spread[r][c] = (c < kOut && !zero) ? (in_range ? sqrtf(var) * os : __builtin_huge_valf())
-                                       : 0.f;
+    const float sd = (c < kOut && !zero) ? (in_range ? sqrtf(var) * os : __builtin_huge_valf())
+                                         : 0.f;
+    if constexpr (CAL) {
+      if (c < kOut && live) {
+        float lin = nm.lin[0][c];
+#pragma unroll
+        for (int i = 0; i < kFeat; ++i) lin = fmaf(nm.lin[1 + i][c], xs[r][i], lin);
+        a.cal_mean[c * a.ld + p] = double(lin) + double(mean[v] * nm.out_std[c] + nm.out_mean[c]);
+        a.cal_spread[c * a.ld + p] = zero ? -1.f : sd;
+      }
+    } else {
+      // predicted error of this log output (0 for a threshold zero)
+      spread[r][c] = (c < kOut && !zero) ? nm.err_a[c] + nm.err_k[c] * sd : 0.f;
+    }
+  }
+  if constexpr (CAL) {
+    if (t < kRows && p0 + t < a.npts) a.cal_outside[p0 + t] = oob[t];
+    return;
   }
   __syncthreads();
   if (t < kRows) {
@@ -629,6 +688,9 @@ struct bnsamd_mlp {
   // on its own stream, allocated on first use
   std::vector<Work> ws;
   // fallback workspace (grown on demand)
+  // training domain (bnsamd_mlp_set_domain)
+  bool domain_set = false;
+  bnsamd_mlp_domain domain{};
   int64_t fb_cap = 0;
   int32_t* fb_idx = nullptr;
   int32_t* fb_count = nullptr;
@@ -729,6 +791,7 @@ void backward(bnsamd_mlp* m, Work& W, int e, int B) {
                          W.dH, W.Z[l - 1], W.dZ[l - 1], m->g[e] + kBOff[l - 1]);
     }
   }
+  hipLaunchKernelGGL(k_tie_grad_l0, dim3(1), dim3(kHidden), 0, s, m->g[e]);
 }
 
 void adam(bnsamd_mlp* m, hipStream_t s, int e, float lr) {
@@ -738,6 +801,7 @@ void adam(bnsamd_mlp* m, hipStream_t s, int e, float lr) {
   hipLaunchKernelGGL(k_adam, dim3(unsigned((kParams + 255) / 256)), dim3(256), 0, s,
                      int64_t(kParams), m->w[e], m->g[e], m->m[e], m->v[e], m->wb[e], lr, b1, b2,
                      eps, c1, c2);
+  hipLaunchKernelGGL(k_mirror_l0, dim3(1), dim3(kHidden), 0, s, m->wb[e]);
 }
 
 // buffers of one workspace (maxb rows); false when an allocation fails
@@ -798,6 +862,7 @@ int bnsamd_mlp_create(int device, int members, int max_batch, bnsamd_mlp** out) 
   m->device = device;
   m->members = members;
   m->maxb = max_batch;
+  for (int o = 0; o < kOut; ++o) m->hnorm.err_k[o] = 1.f;  // predicted error = spread
   bool ok = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
   auto alloc = [&](void** p, size_t bytes) {
     ok = ok && hipMalloc(p, bytes) == hipSuccess && hipMemset(*p, 0, bytes) == hipSuccess;
@@ -861,9 +926,16 @@ int bnsamd_mlp_param_count(void) { return kParams; }
 int bnsamd_mlp_set_params(bnsamd_mlp* m, int member, const float* h_params) {
   if (!m || member < 0 || member >= m->members || !h_params) return BNSAMD_ERR_INVALID;
   (void)hipSetDevice(m->device);
+  // the FP32 master of the first layer's low-part rows stays zero
+  std::vector<float> hp(h_params, h_params + kParams);
+  for (int i = kFeat * kHidden; i < 2 * kFeat * kHidden; ++i) hp[kWOff[0] + i] = 0.f;
   std::vector<uint16_t> hb(kParams);
-  for (int i = 0; i < kParams; ++i) hb[i] = f2bf(h_params[i]);
-  if (hipMemcpy(m->w[member], h_params, sizeof(float) * kParams, hipMemcpyHostToDevice) !=
+  for (int i = 0; i < kParams; ++i) hb[i] = f2bf(hp[i]);
+  // the first layer's low-part rows: the feature rows' weights
+  for (int i = 0; i < kFeat; ++i)
+    for (int n = 0; n < kHidden; ++n)
+      hb[kWOff[0] + (kFeat + i) * kHidden + n] = hb[kWOff[0] + i * kHidden + n];
+  if (hipMemcpy(m->w[member], hp.data(), sizeof(float) * kParams, hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(m->wb[member], hb.data(), sizeof(uint16_t) * kParams, hipMemcpyHostToDevice) !=
           hipSuccess ||
@@ -888,8 +960,12 @@ int bnsamd_mlp_get_params(bnsamd_mlp* m, int member, float* h_params) {
 int bnsamd_mlp_set_norm(bnsamd_mlp* m, const bnsamd_mlp_norm* norm) {
   if (!m || !norm) return BNSAMD_ERR_INVALID;
   (void)hipSetDevice(m->device);
-  m->hnorm = *norm;
-  if (hipMemcpy(m->norm, norm, sizeof(*norm), hipMemcpyHostToDevice) != hipSuccess)
+  // the acceptance coefficients are bnsamd_mlp_set_acceptance's
+  bnsamd_mlp_norm h = *norm;
+  std::memcpy(h.err_a, m->hnorm.err_a, sizeof(h.err_a));
+  std::memcpy(h.err_k, m->hnorm.err_k, sizeof(h.err_k));
+  m->hnorm = h;
+  if (hipMemcpy(m->norm, &m->hnorm, sizeof(m->hnorm), hipMemcpyHostToDevice) != hipSuccess)
     return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_set_norm: copy");
   return BNSAMD_OK;
 }
@@ -1050,36 +1126,133 @@ int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, fl
   return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_predict");
 }
 
+int bnsamd_mlp_set_domain(bnsamd_mlp* m, const bnsamd_mlp_domain* d) {
+  if (!m || !d) return BNSAMD_ERR_INVALID;
+  if (d->nquad < 1 || d->nquad > 32 || (d->stim_abs != 0 && d->stim_abs != 1))
+    return mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_mlp_set_domain: invalid domain");
+  m->domain = *d;
+  m->domain_set = true;
+  return BNSAMD_OK;
+}
+
+int bnsamd_mlp_set_acceptance(bnsamd_mlp* m, const float* err_a, const float* err_k) {
+  if (!m || !err_a || !err_k) return BNSAMD_ERR_INVALID;
+  for (int o = 0; o < kOut; ++o)
+    if (!(err_a[o] >= 0.f) || !(err_k[o] >= 0.f))
+      return mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_mlp_set_acceptance: negative coefficient");
+  (void)hipSetDevice(m->device);
+  std::memcpy(m->hnorm.err_a, err_a, sizeof(float) * kOut);
+  std::memcpy(m->hnorm.err_k, err_k, sizeof(float) * kOut);
+  // the norm block may be read by an inference still queued on a stream
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(m->norm, &m->hnorm, sizeof(m->hnorm), hipMemcpyHostToDevice) != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_set_acceptance: copy");
+  return BNSAMD_OK;
+}
+
+namespace {
+// inference arguments shared by the learned form and the raw predictions;
+// packs the weights when training changed them
+int infer_setup(bnsamd_mlp* m, hipStream_t s, int64_t npts, int64_t ld, const double* d_pts,
+                InferArgs& a, const char* who) {
+  // training runs on the MLP's own streams: parameters are final before use
+  if (sync_all(m) != hipSuccess) return mlp_fail(m, BNSAMD_ERR_HIP, std::string(who) + ": training stream");
+  if (!m->packed_valid) {
+    for (int e = 0; e < m->members; ++e)
+      for (int l = 0; l < kLayers; ++l) {
+        int64_t n = int64_t(kLayerK[l]) * kLayerN[l];
+        hipLaunchKernelGGL(k_pack, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, kLayerK[l],
+                           kLayerN[l], m->w[e] + kWOff[l], m->wp[e] + kPOff[l], l == 0 ? 1 : 0);
+      }
+    m->packed_valid = true;
+  }
+  a = InferArgs{};
+  a.npts = npts;
+  a.ld = ld;
+  a.pts = d_pts;
+  a.norm = reinterpret_cast<const bnsamd_mlp_norm*>(m->norm);
+  for (int e = 0; e < m->members; ++e) {
+    a.wp[e] = m->wp[e];
+    a.w[e] = m->w[e];
+  }
+  a.members = m->members;
+  a.dU = m->domain.dU;
+  a.dm_eff = m->domain.dm_eff;
+  return BNSAMD_OK;
+}
+
+bool same_flags(const bnsamd_flags& x, const bnsamd_flags& y) {
+  return x.use_abs_em == y.use_abs_em && x.use_pair == y.use_pair && x.use_brem == y.use_brem &&
+         x.use_inelastic_scatt == y.use_inelastic_scatt && x.use_iso == y.use_iso;
+}
+bool same_params(const bnsamd_params& x, const bnsamd_params& y) {
+  return x.use_dU == y.use_dU && x.use_dm_eff == y.use_dm_eff && x.use_WM_ab == y.use_WM_ab &&
+         x.use_WM_sc == y.use_WM_sc && x.use_decay == y.use_decay &&
+         x.brem_implementation == y.brem_implementation &&
+         x.use_NN_medium_corr == y.use_NN_medium_corr && x.neglect_blocking == y.neglect_blocking;
+}
+}  // namespace
+
+int bnsamd_mlp_infer(bnsamd_context* ctx, bnsamd_mlp* m, int64_t npts, int64_t ld,
+                     const double* d_pts, double* d_logmean, float* d_spread, int32_t* d_outside) {
+  if (!ctx || !m || npts < 0 || ld < npts || !d_pts || !d_logmean || !d_spread || !d_outside ||
+      npts > int64_t(0x7fffffff))
+    return m ? mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_mlp_infer: invalid argument")
+             : BNSAMD_ERR_INVALID;
+  if (!m->domain_set)
+    return mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_mlp_infer: training domain not set");
+  if (npts == 0) return BNSAMD_OK;
+  (void)hipSetDevice(m->device);
+  hipStream_t s = static_cast<hipStream_t>(bnsamd_get_stream(ctx));
+  InferArgs a;
+  if (int rc = infer_setup(m, s, npts, ld, d_pts, a, "bnsamd_mlp_infer")) return rc;
+  a.threshold = 0.f;
+  a.cal_mean = d_logmean;
+  a.cal_spread = d_spread;
+  a.cal_outside = d_outside;
+  hipLaunchKernelGGL(k_infer<true>, dim3(unsigned((npts + kRows - 1) / kRows)), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_infer: launch");
+}
+
 int bnsamd_surrogate_m1_opacities(bnsamd_context* ctx, bnsamd_mlp* m, int64_t npts, int64_t ld,
-                                   const double* d_pts, float spread_threshold,
-                                   const bnsamd_flags* flags, const bnsamd_params* params,
-                                   int stim_abs, double* d_out, int64_t* n_fallback) {
+                                  const double* d_pts, float error_bound,
+                                  const bnsamd_flags* flags, const bnsamd_params* params,
+                                  int stim_abs, double* d_out, int64_t* n_fallback) {
   if (!ctx || !m || npts < 0 || ld < npts || !d_pts || !d_out || !flags || !params ||
       npts > int64_t(0x7fffffff))
     return m ? mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_surrogate_m1_opacities: invalid argument")
              : BNSAMD_ERR_INVALID;
+  if (!m->domain_set)
+    return mlp_fail(m, BNSAMD_ERR_INVALID,
+                    "bnsamd_surrogate_m1_opacities: training domain not set");
   if (npts == 0) {
     if (n_fallback) *n_fallback = 0;
     return BNSAMD_OK;
   }
   (void)hipSetDevice(m->device);
   hipStream_t s = static_cast<hipStream_t>(bnsamd_get_stream(ctx));
-  // training runs on the MLP's own streams: parameters are final before use
-  if (sync_all(m) != hipSuccess)
-    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate_m1_opacities: training stream");
-  if (!m->packed_valid) {
-    for (int e = 0; e < m->members; ++e)
-      for (int l = 0; l < kLayers; ++l) {
-        int64_t n = int64_t(kLayerK[l]) * kLayerN[l];
-        hipLaunchKernelGGL(k_pack, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, kLayerK[l],
-                           kLayerN[l], m->w[e] + kWOff[l], m->wp[e] + kPOff[l]);
-      }
-    m->packed_valid = true;
+  // a call outside the configuration the targets were made with: every
+  // point through the FP64 path
+  int n1 = 0, n2 = 0;
+  if (!bns::context_quadrature_orders(ctx, &n1, &n2))
+    return mlp_fail(m, BNSAMD_ERR_NO_QUADRATURE, "bnsamd_set_quadrature not called");
+  const bnsamd_mlp_domain& dm = m->domain;
+  if (!same_flags(*flags, dm.flags) || !same_params(*params, dm.params) ||
+      stim_abs != dm.stim_abs || n1 != dm.nquad || n2 != dm.nquad) {
+    int rc = bnsamd_m1_opacities(ctx, npts, ld, d_pts, nullptr, flags, params, stim_abs,
+                                 BNSAMD_FP64, d_out);
+    if (rc != BNSAMD_OK) return mlp_fail(m, rc, std::string("fp64 path: ") + bnsamd_last_error(ctx));
+    if (hipStreamSynchronize(s) != hipSuccess)
+      return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate_m1_opacities: fp64 path");
+    if (n_fallback) *n_fallback = npts;
+    return BNSAMD_OK;
   }
   if (npts > m->fb_cap) {
     auto fr = [](void* p) {
       if (p) (void)hipFree(p);
     };
+    // the previous call's gather / scatter have finished (the call is synchronous)
     fr(m->fb_idx);
     fr(m->fb_count);
     fr(m->fb_pts);
@@ -1095,22 +1268,15 @@ int bnsamd_surrogate_m1_opacities(bnsamd_context* ctx, bnsamd_mlp* m, int64_t np
       return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_surrogate_m1_opacities: workspace");
     m->fb_cap = npts;
   }
-  InferArgs a{};
-  a.npts = npts;
-  a.ld = ld;
-  a.pts = d_pts;
-  a.norm = reinterpret_cast<const bnsamd_mlp_norm*>(m->norm);
-  for (int e = 0; e < m->members; ++e) {
-    a.wp[e] = m->wp[e];
-    a.w[e] = m->w[e];
-  }
-  a.members = m->members;
-  a.threshold = spread_threshold;
+  InferArgs a;
+  if (int rc = infer_setup(m, s, npts, ld, d_pts, a, "bnsamd_surrogate_m1_opacities")) return rc;
+  a.threshold = error_bound;
   a.out = d_out;
   a.fallback_idx = m->fb_idx;
   a.fallback_count = m->fb_count;
   hipError_t e = hipMemsetAsync(m->fb_count, 0, sizeof(int32_t), s);
-  hipLaunchKernelGGL(k_infer, dim3(unsigned((npts + kRows - 1) / kRows)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_infer<false>, dim3(unsigned((npts + kRows - 1) / kRows)), dim3(256), 0, s,
+                     a);
   int32_t nfb = 0;
   if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess)
@@ -1127,10 +1293,13 @@ int bnsamd_surrogate_m1_opacities(bnsamd_context* ctx, bnsamd_mlp* m, int64_t np
       return mlp_fail(m, rc, std::string("fallback: ") + bnsamd_last_error(ctx));
     hipLaunchKernelGGL(k_scatter_out, dim3(g), dim3(256), 0, s, nfb, m->fb_idx, m->fb_out, d_out,
                        ld);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+      return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate_m1_opacities: fallback");
   }
   if (n_fallback) *n_fallback = nfb;
-  e = hipGetLastError();
-  return e == hipSuccess ? BNSAMD_OK : mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_surrogate: fallback");
+  return BNSAMD_OK;
 }
 
 }  // extern "C"

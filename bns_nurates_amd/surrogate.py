@@ -31,7 +31,13 @@ class MlpNorm(C.Structure):
                 ("out_mean", C.c_float * OUT), ("out_std", C.c_float * OUT),
                 ("box_lo", C.c_float * FEAT), ("box_hi", C.c_float * FEAT),
                 ("lin", (C.c_float * OUT) * (FEAT + 1)),
-                ("res_lo", C.c_float * 15), ("res_hi", C.c_float * 15)]
+                ("res_lo", C.c_float * 15), ("res_hi", C.c_float * 15),
+                ("err_a", C.c_float * 15), ("err_k", C.c_float * 15)]
+
+
+class MlpDomain(C.Structure):
+    _fields_ = [("flags", api._Flags), ("params", api._Params), ("stim_abs", C.c_int),
+                ("nquad", C.c_int), ("dU", C.c_double), ("dm_eff", C.c_double)]
 
 
 def features(pts: np.ndarray) -> np.ndarray:
@@ -165,7 +171,8 @@ class Ensemble:
         n = MlpNorm()
         for name, v in (("feat_mean", feat_mean), ("feat_std", feat_std), ("out_mean", out_mean),
                         ("out_std", out_std), ("box_lo", box_lo), ("box_hi", box_hi),
-                        ("res_lo", res_lo), ("res_hi", res_hi)):
+                        ("res_lo", res_lo), ("res_hi", res_hi),
+                        ("err_a", np.zeros(OUT)), ("err_k", np.ones(OUT))):
             arr = getattr(n, name)
             for i, x in enumerate(np.asarray(v, np.float64)):
                 arr[i] = float(x)
@@ -207,11 +214,47 @@ class Ensemble:
         self._check(self.lib.bnsamd_mlp_predict(self.h, member, x.shape[0], _fptr(x), _fptr(y)),
                     "bnsamd_mlp_predict")
 
-    def m1_opacities(self, ctx, pts, out, threshold: float, flags=None, params=None,
+    def set_domain(self, flags, params, stim_abs: int, nquad: int, dU: float, dm_eff: float):
+        """The configuration of the training targets (bnsamd_mlp_set_domain)."""
+        d = MlpDomain(api.as_flags(flags).c(), api.as_params(params).c(), int(stim_abs),
+                      int(nquad), float(dU), float(dm_eff))
+        self._check(self.lib.bnsamd_mlp_set_domain(self.h, C.byref(d)), "bnsamd_mlp_set_domain")
+        self.domain = d
+
+    def set_acceptance(self, err_a, err_k):
+        """Predicted log error err_a + err_k * spread per output
+        (bnsamd_mlp_set_acceptance)."""
+        a = np.ascontiguousarray(err_a, np.float32)
+        k = np.ascontiguousarray(err_k, np.float32)
+        assert a.shape == (OUT,) and k.shape == (OUT,)
+        self._check(self.lib.bnsamd_mlp_set_acceptance(self.h, a.ctypes.data, k.ctypes.data),
+                    "bnsamd_mlp_set_acceptance")
+        self.err_a, self.err_k = a.copy(), k.copy()
+
+    def infer(self, ctx, pts):
+        """Raw predictions on a device point block pts [21, >= n]: mean log
+        outputs [15, n] (float64), their spreads [15, n] (float32) and the
+        outside-the-box-or-domain flags [n] (int32), device tensors."""
+        import torch
+        n = pts.shape[1]
+        pp, ld = api._tensor_ptr(pts, 21, n, "pts")
+        ctx._follow_torch_stream()
+        mean = torch.empty((OUT, ld), dtype=torch.float64, device=pts.device)
+        spread = torch.empty((OUT, ld), dtype=torch.float32, device=pts.device)
+        outside = torch.empty(ld, dtype=torch.int32, device=pts.device)
+        self._check(self.lib.bnsamd_mlp_infer(ctx.h, self.h, n, ld, C.c_void_p(pp),
+                                              C.c_void_p(mean.data_ptr()),
+                                              C.c_void_p(spread.data_ptr()),
+                                              C.c_void_p(outside.data_ptr())), "bnsamd_mlp_infer")
+        return mean[:, :n], spread[:, :n], outside[:n]
+
+    def m1_opacities(self, ctx, pts, out, error_bound: float, flags=None, params=None,
                      stim_abs: int = 1) -> int:
         """The learned path on a device point block pts [21, >= n] into out
-        [20, >= n] (torch float64 tensors, ctx's quadrature for the fallback);
-        returns the number of points recomputed by the FP64 path."""
+        [20, >= n] (torch float64 tensors, ctx's quadrature for the fallback):
+        points whose predicted log error exceeds error_bound on any output
+        (log1p(tol) after calibrate(..., tol)) are recomputed by the FP64
+        path; returns their number.  Synchronous."""
         n = pts.shape[1]
         pp, ld = api._tensor_ptr(pts, 21, n, "pts")
         op, ldo = api._tensor_ptr(out, 20, n, "out")
@@ -221,20 +264,37 @@ class Ensemble:
         fl, pr = api.as_flags(flags).c(), api.as_params(params).c()
         nfb = C.c_int64()
         self._check(self.lib.bnsamd_surrogate_m1_opacities(
-            ctx.h, self.h, n, ld, C.c_void_p(pp), C.c_float(threshold), C.byref(fl), C.byref(pr),
-            int(stim_abs), C.c_void_p(op), C.byref(nfb)), "bnsamd_surrogate_m1_opacities")
+            ctx.h, self.h, n, ld, C.c_void_p(pp), C.c_float(error_bound), C.byref(fl),
+            C.byref(pr), int(stim_abs), C.c_void_p(op), C.byref(nfb)),
+            "bnsamd_surrogate_m1_opacities")
         return int(nfb.value)
+
+
+def check_domain_rules(pts: np.ndarray):
+    """The point-block rules of the training domain (bnsamd_mlp_domain):
+    anux moments equal to nux's, yn = 1 - yp, one dU and one dm_eff value.
+    Returns (dU, dm_eff)."""
+    for f in (W.P_N0, W.P_J0, W.P_CHI0):
+        assert np.array_equal(pts[f + 3], pts[f + 2]), "anux moments must equal nux's"
+    assert np.array_equal(pts[W.P_YN], 1.0 - pts[W.P_YP]), "yn must be 1 - yp"
+    du, dm = np.unique(pts[W.P_DU]), np.unique(pts[W.P_DMEFF])
+    assert du.size == 1 and dm.size == 1, "dU and dm_eff must be constant"
+    return float(du[0]), float(dm[0])
 
 
 def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs: int = 8,
                    batch: int = 4096, lr0: float = 4e-3, lr1: float = 2e-5, seed: int = 1000,
-                   chunk: int = 1 << 21, partial_targets: bool = False,
+                   chunk: int = 1 << 21, partial_targets: bool = False, ln_loss: bool = False,
                    log=None) -> "Ensemble":
     """An ensemble trained on npts seeded points (workload distributions,
-    seed `seed`) with targets from the FP64 path at quadrature order nquad;
-    member e starts from its own initialisation and data order."""
+    seed `seed`) with targets from the FP64 path at quadrature order nquad
+    (all reactions, stimulated absorption); member e starts from its own
+    initialisation and data order.  ln_loss: the residuals share one scale,
+    so that the loss is the mean squared error in natural-log units (the
+    unit of the acceptance bar) instead of per-output standardised units."""
     import torch
     pts = W.make_points(npts, seed=seed)
+    dU, dm_eff = check_domain_rules(pts)
     ctx.set_quadrature(nquad)
     outs = []
     for a in range(0, npts, chunk):
@@ -265,11 +325,14 @@ def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs:
     res = t - X1 @ A
     del X1
     tm, ts = np.nanmean(res, 0), np.nanstd(res, 0)
+    if ln_loss:
+        ts = np.full_like(ts, np.sqrt(np.mean(ts ** 2)))
     ys = (res - tm) / ts
     x = torch.from_numpy(xn.astype(np.float32)).cuda()
     y = torch.from_numpy(ys.astype(np.float32)).cuda()
     ens = Ensemble(ctx.device, members=members, max_batch=batch)
     ens.set_norm(fm, fs, tm, ts, f.min(0), f.max(0), A, np.nanmin(ys, 0), np.nanmax(ys, 0))
+    ens.set_domain(W.FLAGS_ALL, params, 1, nquad, dU, dm_eff)
     ens.lin = A
     ens.n_train, ens.history = int(ok.sum()), []
     # each member from its own initialisation and data order, all trained at
@@ -285,3 +348,195 @@ def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs:
         if log:
             log(f"surrogate member {e}: epoch losses {np.array2string(hist[e], precision=5)}")
     return ens
+
+
+# ---- acceptance rule ------------------------------------------------------------
+# The FP64 path's own rounding noise: outputs formed by cancellation (pair and
+# NES/NPS rates at large degeneracy, some stimulated-absorption opacities)
+# carry rounding errors far above 1e-2 of their value, and can even come out
+# negative.  A relative tolerance against such a value judges nothing, so an
+# accepted output violates the bar when |pred - ref| > tol |ref| + NOISE_K D,
+# D the FP64 path's change under input perturbations of PERTURB (relative)
+# in T and mu_e: far above an ulp, so that the roundings change, and far
+# below any tolerance in its effect on the true value.
+PERTURB = 2.0 ** -46
+NOISE_K = 16.0
+
+
+def fp64_noise(ctx, d, params, ref, flags=None):
+    """Noise estimate D [20, n] of the FP64 outputs ref of the device point
+    block d (ctx's quadrature): the largest change of each output when T or
+    mu_e is scaled by 1 +- PERTURB."""
+    import torch
+    flags = W.FLAGS_ALL if flags is None else flags
+    n = d.shape[1]
+    D = torch.zeros_like(ref[:, :n])
+    for field, f in ((W.P_T, 1.0 + PERTURB), (W.P_MUE, 1.0 - PERTURB)):
+        dp = d.clone()
+        dp[field] *= f
+        r2 = ctx.compute_m1_opacities(dp, flags=flags, params=params)
+        D = torch.maximum(D, (r2[:, :n] - ref[:, :n]).abs())
+        del dp, r2
+    return D
+
+
+def violations(pred: np.ndarray, ref: np.ndarray, noise: np.ndarray, tol: float) -> np.ndarray:
+    """[15, n] True where an output of the learned form misses the bar
+    |pred - ref| > tol |ref| + NOISE_K D (rows OUT_ROWS of the output
+    blocks; exact zeros on both sides pass)."""
+    p, r, D = pred[OUT_ROWS], ref[OUT_ROWS], noise[OUT_ROWS]
+    with np.errstate(invalid="ignore"):
+        bad = np.abs(p - r) > tol * np.abs(r) + NOISE_K * D
+    return bad | ~np.isfinite(p)
+
+
+def prediction_errors(mean: np.ndarray, spread: np.ndarray, outside: np.ndarray,
+                      ref: np.ndarray, noise: np.ndarray, tol: float):
+    """Per output and point: the log error |mean - log ref| where the FP64
+    value is positive and determined to tol (NOISE_K D <= tol |ref|; NaN
+    elsewhere: such outputs do not shape the error model), the bar's verdict
+    on exp(mean) (violations()), the spread, and the mask of points inside
+    the training box and domain.  Threshold zeros (spread < 0) are exact."""
+    t = _log_targets(ref)
+    r, D = ref[OUT_ROWS], noise[OUT_ROWS]
+    judged = (r > 0) & (NOISE_K * D <= tol * np.abs(r))
+    with np.errstate(invalid="ignore", over="ignore"):
+        e = np.where(judged, np.abs(mean - t), np.nan)
+        pred = np.zeros_like(ref)
+        pred[OUT_ROWS] = np.exp(mean)
+    zero_pred = spread < 0.0
+    pred[OUT_ROWS] = np.where(zero_pred, 0.0, pred[OUT_ROWS])
+    bad = violations(pred, ref, noise, tol)
+    e = np.where(zero_pred, 0.0, e)
+    return e, bad, spread.astype(np.float64), outside == 0
+
+
+def _log_targets(ref: np.ndarray) -> np.ndarray:
+    """Natural logs [15, n] of the FP64 outputs, NaN where an output is not
+    positive."""
+    o = ref[OUT_ROWS]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(o > 0, np.log(np.where(o > 0, o, 1.0)), np.nan)
+
+
+def clopper_pearson_upper(k: int, n: int, conf: float = 0.95) -> float:
+    """One-sided upper confidence bound of a binomial proportion."""
+    if n == 0 or k >= n:
+        return 1.0
+    from scipy.stats import beta
+    return float(beta.ppf(conf, k + 1, n - k))
+
+
+def calibrate(ens: "Ensemble", ctx, tol: float, params, nquad: int, ncal: int = 1 << 20,
+              seed: int = 2000, coverage: float = 0.999, q: float = 0.999, log=None,
+              data=None) -> dict:
+    """Calibrates the acceptance rule for a relative tolerance tol on every
+    output and sets it on the ensemble; the learned form is then called with
+    error_bound = log1p(tol).
+
+    Two disjoint seeded splits of ncal points from the workload
+    distributions, with FP64 targets (and their noise estimate, fp64_noise)
+    from the same path and quadrature as the training targets.  Split A
+    fixes, per output o, the shape of the predicted error a_o + k_o s_o =
+    k_o (s_o + delta_o): delta_o among a few multiples of log1p(tol), k_o the
+    q-quantile of e_o / (s_o + delta_o) over the outputs the FP64 value
+    determines to tol, choosing the delta_o that accepts most points for
+    that output alone.  Split B fixes a common scale g: the smallest g on a
+    geometric grid for which every larger g keeps the fraction of accepted
+    points with any output beyond the bar (violations()) at or below
+    (1 - coverage) / 2.  Returns the statistics of both splits (accepted
+    fraction, violations among the accepted, and the 95% Clopper-Pearson
+    upper bound of the violation rate).  data: the splits' (mean, spread,
+    outside, ref, noise) from calibration_data(), to share them between
+    tolerances."""
+    b = float(np.log1p(tol))
+    if data is None:
+        data = calibration_data(ens, ctx, params, nquad, 2 * ncal, seed)
+    mean, spread, outside, ref, noise = data
+    n = ref.shape[1]
+    e, bad, sd, inside = prediction_errors(mean, spread, outside, ref, noise, tol)
+    A, B = slice(0, n // 2), slice(n // 2, n)
+    eA, sA, iA = e[:, A], sd[:, A], inside[A]
+    err_a, err_k = np.zeros(OUT), np.zeros(OUT)
+    per_out = []
+    for o in range(OUT):
+        live = iA & (sA[o] >= 0) & np.isfinite(sA[o]) & ~np.isnan(eA[o])
+        per_out.append({"judged": round(float(live.sum() / max(iA.sum(), 1)), 4),
+                        "median_err": float(f"{np.median(eA[o, live]):.3g}") if live.any()
+                        else None,
+                        "median_spread": float(f"{np.median(sA[o, live]):.3g}") if live.any()
+                        else None})
+        if not live.any():
+            err_a[o], err_k[o] = 0.0, 1.0
+            continue
+        best = None
+        for frac in (0.0, 0.03, 0.1, 0.3, 1.0):
+            delta = frac * b
+            with np.errstate(divide="ignore", invalid="ignore"):
+                r = np.where(eA[o, live] == 0, 0.0, eA[o, live] / (sA[o, live] + delta))
+            r = np.where(np.isnan(r), np.inf, r)
+            k = float(np.quantile(r, q))
+            if not np.isfinite(k):
+                continue
+            acc = float(np.mean(k * (sA[o, live] + delta) <= b))
+            if best is None or acc > best[0]:
+                best = (acc, k, delta)
+        if best is None:  # no finite ratio: this output never accepts
+            err_a[o], err_k[o] = np.inf, np.inf
+        else:
+            err_a[o], err_k[o] = best[1] * best[2], best[1]
+
+    def rule_stats(bb, ss, ins, g):
+        with np.errstate(invalid="ignore", over="ignore"):
+            pe = np.where(ss < 0, 0.0, g * (err_a[:, None] + err_k[:, None] * np.maximum(ss, 0)))
+        acc = ins & np.all(pe <= b, axis=0)
+        viol = acc & np.any(bb, axis=0)
+        return acc, viol
+
+    bB, sB, iB = bad[:, B], sd[:, B], inside[B]
+    g_ok = np.inf
+    for g in np.geomspace(1e-2, 1e3, 121)[::-1]:  # from the strictest rule to the loosest
+        acc, viol = rule_stats(bB, sB, iB, g)
+        if acc.sum() and viol.sum() / acc.sum() > (1.0 - coverage) / 2:
+            break
+        g_ok = g
+    scale = g_ok if np.isfinite(g_ok) else np.inf
+    with np.errstate(invalid="ignore", over="ignore"):
+        a_set = np.nan_to_num(scale * err_a, nan=np.inf)
+        k_set = np.nan_to_num(scale * err_k, nan=np.inf)
+    # a coefficient beyond float range never accepts that output
+    a_set = np.where(np.isfinite(a_set) & (a_set < 3e38), a_set, 3e38).astype(np.float32)
+    k_set = np.where(np.isfinite(k_set) & (k_set < 3e38), k_set, 3e38).astype(np.float32)
+    ens.set_acceptance(a_set, k_set)
+    for o in range(OUT):
+        per_out[o]["err_a"] = float(f"{a_set[o]:.3g}")
+        per_out[o]["err_k"] = float(f"{k_set[o]:.3g}")
+    rep = {"tol": tol, "error_bound_log": b, "scale_g": float(g_ok),
+           "coverage_target": coverage, "calibration_points": n,
+           "bar": f"|pred - ref| <= tol |ref| + {NOISE_K:g} D (D: FP64 change under "
+                  f"2^-46 perturbations of T and mu_e)",
+           "inside_frac": round(float(inside.mean()), 4), "per_output": per_out}
+    for tag, (bb, ss, ins) in (("split_A", (bad[:, A], sA, iA)), ("split_B", (bB, sB, iB))):
+        acc, viol = rule_stats(bb, ss, ins, scale)
+        na, nv = int(acc.sum()), int(viol.sum())
+        rep[tag] = {"accepted_frac": round(na / len(ins), 5), "accepted": na, "violations": nv,
+                    "violation_rate_ucb95": float(f"{clopper_pearson_upper(nv, na):.3g}")}
+    if log:
+        log(f"calibrate tol={tol}: g={g_ok:.3g}, {rep['split_B']}")
+    return rep
+
+
+def calibration_data(ens: "Ensemble", ctx, params, nquad: int, npts: int, seed: int):
+    """Raw predictions and FP64 targets on npts seeded workload points:
+    (mean [15, n], spread [15, n], outside [n], ref [20, n], noise [20, n]),
+    host arrays."""
+    import torch
+    pts = W.make_points(npts, seed=seed)
+    ctx.set_quadrature(nquad)
+    d = torch.from_numpy(pts).cuda()
+    ref = ctx.compute_m1_opacities(d, flags=W.FLAGS_ALL, params=params)
+    noise = fp64_noise(ctx, d, params, ref)
+    mean, spread, outside = (t.cpu().numpy() for t in ens.infer(ctx, d))
+    out = (mean, spread, outside, ref.cpu().numpy(), noise.cpu().numpy())
+    del d, ref, noise
+    return out

@@ -87,8 +87,12 @@ typedef struct bnsamd_context bnsamd_context;
 /* ---- context --------------------------------------------------------------
  * One context per device; it owns a stream (used until bnsamd_set_stream
  * selects another; NULL selects HIP's default stream), the resident
- * quadrature tables and the staging buffers of the host entry point.  Calls on one context are serialised by
- * an internal lock; results never depend on how points are batched. */
+ * quadrature tables and the staging buffers of the host entry point.  Calls
+ * on one context are serialised by an internal lock; results never depend on
+ * how points are batched.  bnsamd_set_stream waits for the work already
+ * queued on the previous stream before switching, so that buffers the
+ * library reuses (staging, fallback workspaces) are never in use on two
+ * streams at once. */
 int bnsamd_create(int device, bnsamd_context** out);
 void bnsamd_destroy(bnsamd_context* ctx);
 const char* bnsamd_last_error(const bnsamd_context* ctx);
@@ -116,8 +120,9 @@ int bnsamd_set_quadrature(bnsamd_context* ctx, int n1, const double* x1, const d
  * d_distr may be NULL: the distribution parameters are then reconstructed on
  * the device from (n, J, chi) as CalculateDistrParamsFromM1 does
  * (distribution.hpp:323-333, the callers' step at tests/tests.hpp:731).
- * precision: BNSAMD_FP64, or BNSAMD_MIXED (FP32 kernels and integrands, FP64
- * Fermi integrals, detailed balance and final sums). */
+ * precision: BNSAMD_FP64, or BNSAMD_MIXED (the distribution values TotalNuF
+ * and the HR98 channel functions in FP32; Fermi integrals, the pair, BRT06
+ * and NES/NPS kernels, detailed balance, integrands and sums in FP64). */
 int bnsamd_m1_opacities(bnsamd_context* ctx, int64_t npts, int64_t ld, const double* d_pts,
                         const double* d_distr, const bnsamd_flags* flags,
                         const bnsamd_params* params, int stim_abs, int precision,
@@ -258,6 +263,8 @@ typedef struct bnsamd_mlp_norm {
   float box_lo[15], box_hi[15];
   float lin[16][15];
   float res_lo[15], res_hi[15]; /* range of the standardised residual in training */
+  float err_a[15], err_k[15];   /* acceptance coefficients: set by bnsamd_mlp_set_acceptance,
+                                   left unchanged by bnsamd_mlp_set_norm */
 } bnsamd_mlp_norm;
 
 int bnsamd_mlp_create(int device, int members, int max_batch, bnsamd_mlp** out);
@@ -289,16 +296,54 @@ int bnsamd_mlp_train_ensemble(bnsamd_mlp* m, int64_t n, const float* d_x, const 
                               float* h_epoch_loss);
 /* standardised outputs d_y [rows][15] of one member (training-path kernels) */
 int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, float* d_y);
+/* The configuration the ensemble's training targets were made with: every
+ * input of the FP64 path that is not a feature.  A point block evaluated by
+ * the learned form must agree with it: the same flags, parameters and
+ * stimulated-absorption switch, the context's 1D and 2D quadratures of order
+ * nquad, and per point dU and dm_eff equal to the values here, yn = 1 - yp,
+ * and the anux moments (n, J, chi) equal to the nux ones (the features carry
+ * nux only). */
+typedef struct bnsamd_mlp_domain {
+  bnsamd_flags flags;
+  bnsamd_params params;
+  int stim_abs;
+  int nquad;
+  double dU, dm_eff;
+} bnsamd_mlp_domain;
+int bnsamd_mlp_set_domain(bnsamd_mlp* m, const bnsamd_mlp_domain* domain);
+
+/* Acceptance rule of the learned form: the predicted error of log output o
+ * (natural-log units) is err_a[o] + err_k[o] * s_o, s_o the standard
+ * deviation of the members' log predictions.  Until this is called,
+ * err_a = 0 and err_k = 1 (the predicted error is the spread itself).  The
+ * coefficients come from a calibration on held-out points with FP64 targets
+ * (bns_nurates_amd/surrogate.py calibrate). */
+int bnsamd_mlp_set_acceptance(bnsamd_mlp* m, const float* err_a, const float* err_k);
+
+/* The ensemble's raw predictions on npts points (calibration and
+ * diagnostics): d_logmean[15][ld] the mean log output (nue, anue, nux x
+ * eta_0, eta, kappa_0_a, kappa_a, kappa_s), d_spread[15][ld] its spread s_o
+ * (-1 for an output the FP64 path sets to zero below its density threshold,
+ * +inf when the predicted residual leaves its training range), d_outside[ld]
+ * 1 where the point is outside the training box or the domain.  Asynchronous
+ * on the context's stream. */
+int bnsamd_mlp_infer(bnsamd_context* ctx, bnsamd_mlp* m, int64_t npts, int64_t ld,
+                     const double* d_pts, double* d_logmean, float* d_spread, int32_t* d_outside);
+
 /* The learned form of bnsamd_m1_opacities (same point and output blocks, on
- * the context's stream): every member of the ensemble evaluates each point;
- * points whose members disagree by more than spread_threshold (standard
- * deviation of a log output), whose features leave the training box, or
- * whose predicted residual leaves the range seen in training are
- * recomputed by the FP64 path with the context's quadrature (which must be
- * the one the targets were made with), the others take the ensemble mean.
- * The number of recomputed points goes to *n_fallback.  Synchronous. */
+ * the context's stream): every member of the ensemble evaluates each point.
+ * A point is recomputed by the FP64 path when its features leave the
+ * training box, it leaves the training domain (bnsamd_mlp_domain), a
+ * predicted residual leaves the range seen in training, or the predicted
+ * error of any output (bnsamd_mlp_set_acceptance) exceeds error_bound
+ * (natural-log units: log1p(tol) for a relative tolerance tol); the other
+ * points take the ensemble mean.  When flags, params, stim_abs or the
+ * context's quadrature orders differ from the training domain, every point
+ * is recomputed.  Fails with BNSAMD_ERR_INVALID before
+ * bnsamd_mlp_set_domain.  The number of recomputed points goes to
+ * *n_fallback.  Synchronous: the outputs are complete on return. */
 int bnsamd_surrogate_m1_opacities(bnsamd_context* ctx, bnsamd_mlp* m, int64_t npts, int64_t ld,
-                                  const double* d_pts, float spread_threshold,
+                                  const double* d_pts, float error_bound,
                                   const bnsamd_flags* flags, const bnsamd_params* params,
                                   int stim_abs, double* d_out, int64_t* n_fallback);
 
