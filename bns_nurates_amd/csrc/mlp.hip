@@ -374,6 +374,8 @@ struct InferArgs {
   int members;
   float threshold;                   // predicted log error that triggers fallback
   double dU, dm_eff;                 // training-domain values of these point fields
+  double mu_np_shift;                // dQ + dU of the beta rates in the training domain
+  double beta_guard;                 // |mu_n - mu_p - mu_np_shift| < beta_guard T: fall back
   double* out;                       // [20][ld]
   int32_t* fallback_idx;             // compacted indices of the points to recompute
   int32_t* fallback_count;
@@ -491,6 +493,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       // values: anux moments equal to nux's, yn = 1 - yp, dU and dm_eff
       in = q[12 * ld] == q[11 * ld] && q[16 * ld] == q[15 * ld] && q[20 * ld] == q[19 * ld] &&
            q[3 * ld] == 1. - q[2 * ld] && q[7 * ld] == a.dU && q[8 * ld] == a.dm_eff;
+      // near the branch surfaces of the beta rates' nucleon factors
+      in = in && !(fabs(q[5 * ld] - q[4 * ld] - a.mu_np_shift) < a.beta_guard * q[1 * ld]);
     } else {
 #pragma unroll
       for (int i = 0; i < kFeat; ++i) f[i] = nm.feat_mean[i];
@@ -1128,7 +1132,8 @@ int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, fl
 
 int bnsamd_mlp_set_domain(bnsamd_mlp* m, const bnsamd_mlp_domain* d) {
   if (!m || !d) return BNSAMD_ERR_INVALID;
-  if (d->nquad < 1 || d->nquad > 32 || (d->stim_abs != 0 && d->stim_abs != 1))
+  if (d->nquad < 1 || d->nquad > 32 || (d->stim_abs != 0 && d->stim_abs != 1) ||
+      !(d->beta_guard >= 0.))
     return mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_mlp_set_domain: invalid domain");
   m->domain = *d;
   m->domain_set = true;
@@ -1178,6 +1183,9 @@ int infer_setup(bnsamd_mlp* m, hipStream_t s, int64_t npts, int64_t ld, const do
   a.members = m->members;
   a.dU = m->domain.dU;
   a.dm_eff = m->domain.dm_eff;
+  const bnsamd_params& pr = m->domain.params;
+  a.mu_np_shift = (pr.use_dm_eff ? m->domain.dm_eff : bns::c::Q) + (pr.use_dU ? m->domain.dU : 0.);
+  a.beta_guard = m->domain.beta_guard;
   return BNSAMD_OK;
 }
 

@@ -37,7 +37,8 @@ class MlpNorm(C.Structure):
 
 class MlpDomain(C.Structure):
     _fields_ = [("flags", api._Flags), ("params", api._Params), ("stim_abs", C.c_int),
-                ("nquad", C.c_int), ("dU", C.c_double), ("dm_eff", C.c_double)]
+                ("nquad", C.c_int), ("dU", C.c_double), ("dm_eff", C.c_double),
+                ("beta_guard", C.c_double)]
 
 
 def features(pts: np.ndarray) -> np.ndarray:
@@ -214,10 +215,11 @@ class Ensemble:
         self._check(self.lib.bnsamd_mlp_predict(self.h, member, x.shape[0], _fptr(x), _fptr(y)),
                     "bnsamd_mlp_predict")
 
-    def set_domain(self, flags, params, stim_abs: int, nquad: int, dU: float, dm_eff: float):
+    def set_domain(self, flags, params, stim_abs: int, nquad: int, dU: float, dm_eff: float,
+                   beta_guard: float = 0.0):
         """The configuration of the training targets (bnsamd_mlp_set_domain)."""
         d = MlpDomain(api.as_flags(flags).c(), api.as_params(params).c(), int(stim_abs),
-                      int(nquad), float(dU), float(dm_eff))
+                      int(nquad), float(dU), float(dm_eff), float(beta_guard))
         self._check(self.lib.bnsamd_mlp_set_domain(self.h, C.byref(d)), "bnsamd_mlp_set_domain")
         self.domain = d
 
@@ -285,16 +287,22 @@ def check_domain_rules(pts: np.ndarray):
 def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs: int = 8,
                    batch: int = 4096, lr0: float = 4e-3, lr1: float = 2e-5, seed: int = 1000,
                    chunk: int = 1 << 21, partial_targets: bool = False, ln_loss: bool = False,
-                   log=None) -> "Ensemble":
+                   beta_guard: float = 0.1, log=None) -> "Ensemble":
     """An ensemble trained on npts seeded points (workload distributions,
     seed `seed`) with targets from the FP64 path at quadrature order nquad
     (all reactions, stimulated absorption); member e starts from its own
     initialisation and data order.  ln_loss: the residuals share one scale,
     so that the loss is the mean squared error in natural-log units (the
-    unit of the acceptance bar) instead of per-output standardised units."""
+    unit of the acceptance bar) instead of per-output standardised units.
+    beta_guard: points with |mu_np| < beta_guard T (mu_np = mu_n - mu_p - dQ -
+    dU of the beta rates) are neither trained on nor accepted: the
+    reference's beta rates jump across the branch surfaces of EtaNNAbs
+    there (formulas_beta_iso.hpp:79-96), which no smooth network fits."""
     import torch
     pts = W.make_points(npts, seed=seed)
     dU, dm_eff = check_domain_rules(pts)
+    prm = api.as_params(params)
+    shift = (dm_eff if prm.use_dm_eff else W.Q_NP) + (dU if prm.use_dU else 0.0)
     ctx.set_quadrature(nquad)
     outs = []
     for a in range(0, npts, chunk):
@@ -315,6 +323,7 @@ def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs:
     if not partial_targets:
         ok &= ~np.any(np.isnan(t), axis=1)
     ok &= np.all(np.isfinite(f), axis=1)
+    ok &= ~(np.abs(pts[W.P_MUN] - pts[W.P_MUP] - shift) < beta_guard * pts[W.P_T])
     f, t = f[ok], t[ok]
     fm, fs = f.mean(0), f.std(0)
     xn = (f - fm) / fs
@@ -332,7 +341,7 @@ def train_ensemble(ctx, npts: int, nquad: int, params, members: int = 4, epochs:
     y = torch.from_numpy(ys.astype(np.float32)).cuda()
     ens = Ensemble(ctx.device, members=members, max_batch=batch)
     ens.set_norm(fm, fs, tm, ts, f.min(0), f.max(0), A, np.nanmin(ys, 0), np.nanmax(ys, 0))
-    ens.set_domain(W.FLAGS_ALL, params, 1, nquad, dU, dm_eff)
+    ens.set_domain(W.FLAGS_ALL, params, 1, nquad, dU, dm_eff, beta_guard)
     ens.lin = A
     ens.n_train, ens.history = int(ok.sum()), []
     # each member from its own initialisation and data order, all trained at
@@ -427,28 +436,43 @@ def clopper_pearson_upper(k: int, n: int, conf: float = 0.95) -> float:
     return float(beta.ppf(conf, k + 1, n - k))
 
 
+def _accept_threshold(score: np.ndarray, bad: np.ndarray, target: float, minacc: int):
+    """The largest score threshold t for which the points with score <= t
+    number at least minacc and have a violation fraction <= target (-inf
+    when there is none)."""
+    o = np.argsort(score, kind="stable")
+    sc, bb = score[o], bad[o]
+    fin = np.isfinite(sc)
+    k = np.arange(1, len(sc) + 1)
+    ok = (np.cumsum(bb) <= target * k) & (k >= minacc) & fin
+    # ties: a threshold accepts every point at its score
+    last_of_tie = np.r_[sc[1:] != sc[:-1], True]
+    ok &= last_of_tie
+    return float(sc[np.nonzero(ok)[0].max()]) if ok.any() else -np.inf
+
+
 def calibrate(ens: "Ensemble", ctx, tol: float, params, nquad: int, ncal: int = 1 << 20,
-              seed: int = 2000, coverage: float = 0.999, q: float = 0.999, log=None,
-              data=None) -> dict:
+              seed: int = 2000, coverage: float = 0.999, q: float = 0.99, minacc: int = 3000,
+              log=None, data=None) -> dict:
     """Calibrates the acceptance rule for a relative tolerance tol on every
     output and sets it on the ensemble; the learned form is then called with
     error_bound = log1p(tol).
 
     Two disjoint seeded splits of ncal points from the workload
     distributions, with FP64 targets (and their noise estimate, fp64_noise)
-    from the same path and quadrature as the training targets.  Split A
-    fixes, per output o, the shape of the predicted error a_o + k_o s_o =
-    k_o (s_o + delta_o): delta_o among a few multiples of log1p(tol), k_o the
-    q-quantile of e_o / (s_o + delta_o) over the outputs the FP64 value
-    determines to tol, choosing the delta_o that accepts most points for
-    that output alone.  Split B fixes a common scale g: the smallest g on a
-    geometric grid for which every larger g keeps the fraction of accepted
-    points with any output beyond the bar (violations()) at or below
-    (1 - coverage) / 2.  Returns the statistics of both splits (accepted
-    fraction, violations among the accepted, and the 95% Clopper-Pearson
-    upper bound of the violation rate).  data: the splits' (mean, spread,
-    outside, ref, noise) from calibration_data(), to share them between
-    tolerances."""
+    from the same path and quadrature as the training targets.  A rule
+    accepts a point when max_o (a_o + k_o s_o) <= t over its outputs (s_o
+    the spread, threshold zeros excepted).  Three shapes (a, k) are tried:
+    the spread itself, the spread over its median per output, and per output
+    the q-quantile ratio of log error to (spread + delta_o) with the delta_o
+    (a multiple of log1p(tol)) that accepts most points for that output
+    alone.  On split A each shape gets the threshold t that accepts most
+    points while at most (1 - coverage) / 2 of them have an output beyond the
+    bar (violations()), and at least minacc are accepted; the shape that
+    accepts most wins, and split B reports it (accepted fraction, violations
+    among the accepted, 95% Clopper-Pearson upper bound of their rate).
+    data: the splits' (mean, spread, outside, ref, noise) from
+    calibration_data(), to share them between tolerances."""
     b = float(np.log1p(tol))
     if data is None:
         data = calibration_data(ens, ctx, params, nquad, 2 * ncal, seed)
@@ -456,73 +480,74 @@ def calibrate(ens: "Ensemble", ctx, tol: float, params, nquad: int, ncal: int = 
     n = ref.shape[1]
     e, bad, sd, inside = prediction_errors(mean, spread, outside, ref, noise, tol)
     A, B = slice(0, n // 2), slice(n // 2, n)
-    eA, sA, iA = e[:, A], sd[:, A], inside[A]
-    err_a, err_k = np.zeros(OUT), np.zeros(OUT)
-    per_out = []
+    sdp = np.where(sd < 0, 0.0, sd)
+    live_all = inside[None, :] & (sd >= 0) & np.isfinite(sd)
+    med = np.array([np.median(sdp[o, A][live_all[o, A]]) if live_all[o, A].any() else 1.0
+                    for o in range(OUT)])
+    shapes = {"spread": (np.zeros(OUT), np.ones(OUT)),
+              "spread_over_median": (np.zeros(OUT), 1.0 / np.maximum(med, 1e-30))}
+    qa, qk = np.zeros(OUT), np.ones(OUT)
     for o in range(OUT):
-        live = iA & (sA[o] >= 0) & np.isfinite(sA[o]) & ~np.isnan(eA[o])
-        per_out.append({"judged": round(float(live.sum() / max(iA.sum(), 1)), 4),
-                        "median_err": float(f"{np.median(eA[o, live]):.3g}") if live.any()
-                        else None,
-                        "median_spread": float(f"{np.median(sA[o, live]):.3g}") if live.any()
-                        else None})
+        live = live_all[o, A] & ~np.isnan(e[o, A])
         if not live.any():
-            err_a[o], err_k[o] = 0.0, 1.0
             continue
+        eo, so = e[o, A][live], sd[o, A][live]
         best = None
         for frac in (0.0, 0.03, 0.1, 0.3, 1.0):
             delta = frac * b
             with np.errstate(divide="ignore", invalid="ignore"):
-                r = np.where(eA[o, live] == 0, 0.0, eA[o, live] / (sA[o, live] + delta))
+                r = np.where(eo == 0, 0.0, eo / (so + delta))
             r = np.where(np.isnan(r), np.inf, r)
             k = float(np.quantile(r, q))
-            if not np.isfinite(k):
+            if not np.isfinite(k) or k <= 0:
                 continue
-            acc = float(np.mean(k * (sA[o, live] + delta) <= b))
+            acc = float(np.mean(k * (so + delta) <= b))
             if best is None or acc > best[0]:
                 best = (acc, k, delta)
-        if best is None:  # no finite ratio: this output never accepts
-            err_a[o], err_k[o] = np.inf, np.inf
-        else:
-            err_a[o], err_k[o] = best[1] * best[2], best[1]
+        if best is not None:
+            qa[o], qk[o] = best[1] * best[2], best[1]
+    shapes["quantile_ratio"] = (qa, qk)
 
-    def rule_stats(bb, ss, ins, g):
+    def score(a_, k_, cols):
         with np.errstate(invalid="ignore", over="ignore"):
-            pe = np.where(ss < 0, 0.0, g * (err_a[:, None] + err_k[:, None] * np.maximum(ss, 0)))
-        acc = ins & np.all(pe <= b, axis=0)
-        viol = acc & np.any(bb, axis=0)
-        return acc, viol
+            pe = np.where(sd[:, cols] < 0, 0.0, a_[:, None] + k_[:, None] * sdp[:, cols])
+        return np.where(inside[cols], pe.max(axis=0), np.inf)
 
-    bB, sB, iB = bad[:, B], sd[:, B], inside[B]
-    g_ok = np.inf
-    for g in np.geomspace(1e-2, 1e3, 121)[::-1]:  # from the strictest rule to the loosest
-        acc, viol = rule_stats(bB, sB, iB, g)
-        if acc.sum() and viol.sum() / acc.sum() > (1.0 - coverage) / 2:
-            break
-        g_ok = g
-    scale = g_ok if np.isfinite(g_ok) else np.inf
-    with np.errstate(invalid="ignore", over="ignore"):
-        a_set = np.nan_to_num(scale * err_a, nan=np.inf)
-        k_set = np.nan_to_num(scale * err_k, nan=np.inf)
-    # a coefficient beyond float range never accepts that output
+    anybad = np.any(bad, axis=0)
+    target = (1.0 - coverage) / 2
+    fits = {}
+    for name, (a_, k_) in shapes.items():
+        sA = score(a_, k_, A)
+        t = _accept_threshold(sA, anybad[A], target, minacc)
+        fits[name] = (t, int((sA <= t).sum()) if np.isfinite(t) else 0)
+    name = max(fits, key=lambda k: fits[k][1])
+    t = fits[name][0]
+    a_, k_ = shapes[name]
+    if np.isfinite(t) and t > 0:
+        a_set, k_set = a_ * b / t, k_ * b / t
+    else:  # nothing accepts: every output's predicted error above any bound
+        a_set, k_set = np.full(OUT, 3e38), np.full(OUT, 3e38)
     a_set = np.where(np.isfinite(a_set) & (a_set < 3e38), a_set, 3e38).astype(np.float32)
     k_set = np.where(np.isfinite(k_set) & (k_set < 3e38), k_set, 3e38).astype(np.float32)
     ens.set_acceptance(a_set, k_set)
-    for o in range(OUT):
-        per_out[o]["err_a"] = float(f"{a_set[o]:.3g}")
-        per_out[o]["err_k"] = float(f"{k_set[o]:.3g}")
-    rep = {"tol": tol, "error_bound_log": b, "scale_g": float(g_ok),
+    rep = {"tol": tol, "error_bound_log": b, "rule": name,
+           "candidates_split_A_accepted": {k: v[1] for k, v in fits.items()},
            "coverage_target": coverage, "calibration_points": n,
            "bar": f"|pred - ref| <= tol |ref| + {NOISE_K:g} D (D: FP64 change under "
                   f"2^-46 perturbations of T and mu_e)",
-           "inside_frac": round(float(inside.mean()), 4), "per_output": per_out}
-    for tag, (bb, ss, ins) in (("split_A", (bad[:, A], sA, iA)), ("split_B", (bB, sB, iB))):
-        acc, viol = rule_stats(bb, ss, ins, scale)
-        na, nv = int(acc.sum()), int(viol.sum())
-        rep[tag] = {"accepted_frac": round(na / len(ins), 5), "accepted": na, "violations": nv,
+           "inside_frac": round(float(inside.mean()), 4),
+           "err_a": [float(f"{v:.3g}") for v in a_set], "err_k": [float(f"{v:.3g}") for v in k_set]}
+    for tag, cols in (("split_A", A), ("split_B", B)):
+        with np.errstate(invalid="ignore", over="ignore"):
+            pe = np.where(sd[:, cols] < 0, 0.0,
+                          a_set[:, None].astype(np.float64) + k_set[:, None] * sdp[:, cols])
+        acc = inside[cols] & np.all(pe <= b, axis=0)
+        na, nv = int(acc.sum()), int((acc & anybad[cols]).sum())
+        rep[tag] = {"accepted_frac": round(na / int(np.sum(np.ones(n, bool)[cols])), 5),
+                    "accepted": na, "violations": nv,
                     "violation_rate_ucb95": float(f"{clopper_pearson_upper(nv, na):.3g}")}
     if log:
-        log(f"calibrate tol={tol}: g={g_ok:.3g}, {rep['split_B']}")
+        log(f"calibrate tol={tol}: rule {name}, {rep['split_B']}")
     return rep
 
 

@@ -302,13 +302,19 @@ int bnsamd_mlp_predict(bnsamd_mlp* m, int member, int rows, const float* d_x, fl
  * stimulated-absorption switch, the context's 1D and 2D quadratures of order
  * nquad, and per point dU and dm_eff equal to the values here, yn = 1 - yp,
  * and the anux moments (n, J, chi) equal to the nux ones (the features carry
- * nux only). */
+ * nux only).  beta_guard: points with |mu_np| < beta_guard * T are left to
+ * the FP64 path, mu_np = mu_n - mu_p - dQ - dU the chemical-potential
+ * difference of the beta rates (formulas_beta_iso.hpp:161-180): the
+ * reference's EtaNNAbs switches branch where mu_np or the sign of
+ * (n_n - n_p) / mu_np changes (formulas_beta_iso.hpp:79-96), and the rates
+ * jump there. */
 typedef struct bnsamd_mlp_domain {
   bnsamd_flags flags;
   bnsamd_params params;
   int stim_abs;
   int nquad;
   double dU, dm_eff;
+  double beta_guard;
 } bnsamd_mlp_domain;
 int bnsamd_mlp_set_domain(bnsamd_mlp* m, const bnsamd_mlp_domain* domain);
 
