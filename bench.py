@@ -576,13 +576,21 @@ def grid_parity(ctx, pts, out, nquad, params, nsample=2048):
     ex = O.parity_excess(got, ref, mag)
     with np.errstate(divide="ignore", invalid="ignore"):
         rel = np.where(got == ref, 0.0, np.abs(got - ref) / np.abs(ref))
-    det = (64 * O.U * np.abs(mag) <= 1e-12 * np.abs(ref)) & (ref != 0)
+    tiers = {}
+    for cut in (1e-11, 1e-10, 1e-9):
+        det = (64 * O.U * np.abs(mag) <= cut * np.abs(ref)) & (ref != 0)
+        tiers[f"{cut:g}"] = {"frac": float(f"{det.mean():.3g}"),
+                             "rel_err_max": float(f"{rel[det].max():.3g}") if det.any() else None}
     return {"sample": f"{nsample} points strided over the grid, all 20 outputs",
             "bar": "|gpu - ref| <= 1e-12 |ref| + 64 u m",
             "bar_excess_max": float(f"{ex.max():.3g}"),
             "rel_err_max_all_outputs": float(f"{rel.max():.3g}"),
-            "fp64_determined_frac": float(f"{det.mean():.3g}"),
-            "rel_err_max_fp64_determined": float(f"{rel[det].max():.3g}") if det.any() else None}
+            "frac_outputs_within_1e-12": float(f"{np.mean(rel <= 1e-12):.4g}"),
+            "fp64_determined": tiers,
+            "note": "fp64_determined[c]: outputs whose FP64 rounding-error bound 64 u m is at most "
+                    "c |ref| (at n = 16 the bound never reaches 1e-12: its best case is ~2e-12), "
+                    "with the GPU's largest relative error on them; the other outputs are "
+                    "cancellation-dominated, where the bar's 64 u m term applies"}
 
 
 def host_roundtrip(ctx, pts, params, passes=3):
