@@ -170,6 +170,8 @@ def load_library():
                                  C.c_int, C.c_int, vp], C.c_int),
         "bnsamd_m1_opacities_host": ([vp, i64, vp, vp, C.POINTER(_Flags), C.POINTER(_Params),
                                       C.c_int, C.c_int, vp], C.c_int),
+        "bnsamd_m1_opacities_host_static": ([vp, i64, vp, vp, C.POINTER(_Flags),
+                                             C.POINTER(_Params), C.c_int, C.c_int, vp], C.c_int),
         "bnsamd_spectral_opacities": ([vp, i64, i64, vp, vp, i64, vp, i64, C.POINTER(_Flags),
                                        C.POINTER(_Params), C.c_int, vp], C.c_int),
         "bnsamd_spectral_opacities_host": ([vp, i64, vp, vp, i64, vp, C.c_int,

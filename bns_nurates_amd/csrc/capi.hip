@@ -27,6 +27,33 @@ struct Tables {
   double* dev = nullptr;
 };
 
+// Staging block of bnsamd_m1_opacities_host_static: point, distribution and
+// output blocks of up to BNSAMD_STATIC_MAX_POINTS points, in device memory
+// that the code object itself holds (no allocation at any call).  Host code
+// reaches it only through the address hipGetSymbolAddress gives per device.
+constexpr size_t kScratchDoubles = size_t(BNSAMD_STATIC_MAX_POINTS) *
+                                   (BNSAMD_PT_NFIELDS + BNSAMD_DI_NFIELDS + BNSAMD_OUT_NFIELDS);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// device address of `scratch` per device, set once per device at context
+// creation with that device current
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+hipError_t scratch_address(int device, double** out) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
+}
+
 }  // namespace
 
 struct bnsamd_context {
@@ -41,6 +68,8 @@ struct bnsamd_context {
   // staging buffers of the host entry point
   double* stage = nullptr;
   size_t stage_doubles = 0;
+  // this device's static staging block (bnsamd_m1_opacities_host_static)
+  double* scratch = nullptr;
 };
 
 namespace {
@@ -282,7 +311,9 @@ int bnsamd_create(int device, bnsamd_context** out) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = scratch_address(device, &c->scratch);  // with the device current
   if (e != hipSuccess) {
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
     return BNSAMD_ERR_HIP;
   }
@@ -437,6 +468,39 @@ int bnsamd_m1_opacities_host(bnsamd_context* c, int64_t npts, const double* h_pt
                      hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "bnsamd_m1_opacities_host: download");
+  return BNSAMD_OK;
+}
+
+int bnsamd_m1_opacities_host_static(bnsamd_context* c, int64_t npts, const double* h_pts,
+                                    const double* h_distr, const bnsamd_flags* flags,
+                                    const bnsamd_params* params, int stim_abs, int precision,
+                                    double* h_out) {
+  if (check_ctx(c)) return BNSAMD_ERR_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  if (npts < 0 || !h_pts || !h_out)
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_m1_opacities_host_static: invalid argument");
+  if (npts > BNSAMD_STATIC_MAX_POINTS)
+    return fail(c, BNSAMD_ERR_INVALID,
+                "bnsamd_m1_opacities_host_static: npts above BNSAMD_STATIC_MAX_POINTS (" +
+                    std::to_string(BNSAMD_STATIC_MAX_POINTS) + ")");
+  if (npts == 0) return BNSAMD_OK;
+  (void)hipSetDevice(c->device);
+  // point block, output block, distribution block at npts leading dimension
+  double* dp = c->scratch;
+  double* dout = dp + size_t(npts) * BNSAMD_PT_NFIELDS;
+  double* dd = h_distr ? dout + size_t(npts) * BNSAMD_OUT_NFIELDS : nullptr;
+  hipError_t e = hipMemcpyAsync(dp, h_pts, sizeof(double) * npts * BNSAMD_PT_NFIELDS,
+                                hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && dd)
+    e = hipMemcpyAsync(dd, h_distr, sizeof(double) * npts * BNSAMD_DI_NFIELDS,
+                       hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "bnsamd_m1_opacities_host_static: upload");
+  int rc = run_gray(c, npts, npts, dp, dd, flags, params, stim_abs, precision, dout);
+  if (rc != BNSAMD_OK) return rc;
+  e = hipMemcpyAsync(h_out, dout, sizeof(double) * npts * BNSAMD_OUT_NFIELDS,
+                     hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "bnsamd_m1_opacities_host_static: download");
   return BNSAMD_OK;
 }
 

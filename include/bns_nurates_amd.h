@@ -135,6 +135,22 @@ int bnsamd_m1_opacities_host(bnsamd_context* ctx, int64_t npts, const double* h_
                              const bnsamd_params* params, int stim_abs, int precision,
                              double* h_out);
 
+/* Same as bnsamd_m1_opacities_host, staged through device memory that the
+ * library's code object holds (a static array sized for
+ * BNSAMD_STATIC_MAX_POINTS points, 9 MB per device, held until the process
+ * ends): the call allocates nothing.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Rejects npts above
+ * BNSAMD_STATIC_MAX_POINTS with BNSAMD_ERR_INVALID.  Every call on one
+ * device shares that device's block (calls on one context are serialised;
+ * calls through two contexts on the same device that overlap race).
+ * Synchronous; ld of the host blocks is npts. */
+#define BNSAMD_STATIC_MAX_POINTS 16384
+int bnsamd_m1_opacities_host_static(bnsamd_context* ctx, int64_t npts, const double* h_pts,
+                                    const double* h_distr, const bnsamd_flags* flags,
+                                    const bnsamd_params* params, int stim_abs, int precision,
+                                    double* h_out);
+
 /* Spectral opacities of npts points at nnu neutrino energies each, device
  * buffers, asynchronous on the context stream.  Replaces
  * ComputeSpectralOpacitiesNotStimulatedAbs (stim_abs = 0) and
