@@ -355,9 +355,21 @@ def bound_distr_from_m1(pts):
     return d, mag
 
 
-def parity_excess(got, ref, mag, rtol=1e-12, k=64.0):
+# k of the parity bar |got - ref| <= rtol |ref| + k U m.  FP64 (rtol <= 1e-12):
+# to first order each double evaluation of the formulas lies within U m of
+# their exact value (Wilkinson's running error bound, error_bound.cpp), so two
+# of them differ by at most 2 U m.  Reduced-precision forms are not covered
+# by that argument; their bar keeps a factor 64 on the FP64 term.
+K_FP64 = 2.0
+K_REDUCED = 64.0
+
+
+def parity_excess(got, ref, mag, rtol=1e-12, k=None):
     """Elementwise |got - ref| / (rtol |ref| + k U m): <= 1 passes the
-    condition-aware parity bar.  Equal values (incl. both NaN) give 0."""
+    condition-aware parity bar (k: K_FP64 when rtol <= 1e-12, else
+    K_REDUCED).  Equal values (incl. both NaN) give 0."""
+    if k is None:
+        k = K_FP64 if rtol <= 1e-12 else K_REDUCED
     got = np.asarray(got, dtype=np.float64)
     ref = np.asarray(ref, dtype=np.float64)
     same = (got == ref) | (np.isnan(got) & np.isnan(ref))

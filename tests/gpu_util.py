@@ -39,7 +39,7 @@ def report(tag, got, ref, names=None):
     return r
 
 
-def check_parity(tag, got, ref, mag, names=None, rtol=1e-12, k=64.0):
+def check_parity(tag, got, ref, mag, names=None, rtol=1e-12, k=None):
     """Condition-aware parity: |got - ref| <= rtol |ref| + k U m, m the
     running error magnitude of the oracle (oracle/error_bound.cpp).  Prints
     the excess and the plain relative error over the well-conditioned

@@ -5,8 +5,10 @@ over libbnsamd.so) on a MI355X and compares with:
 * the reference's own outputs stored in tests/golden (n <= 10), and
 * the CPU oracle (oracle/, bit-exact with the reference) on the same seeded
   inputs, which also covers n = 16 where the reference cannot run.
-FP64 bar, per output: |gpu - ref| <= 1e-12 |ref| + 64 u m, where m is the
-running error magnitude of that output (oracle/error_bound.cpp, u = 2^-53).
+FP64 bar, per output: |gpu - ref| <= 1e-12 |ref| + 2 u m, where m is the
+running error magnitude of that output (oracle/error_bound.cpp, u = 2^-53):
+to first order each double evaluation lies within u m of the exact value of
+the formulas, so two of them differ by at most 2 u m.
 For well-conditioned outputs this is the plain 1e-12 relative bar; outputs
 formed by cancellation (pair and NES rates at large degeneracy, where the
 reference itself yields rounding-noise and even negative rates, see

@@ -6,7 +6,7 @@ and compares with the reference's own outputs stored in tests/golden (the six
 points of the reference's spectral test; seeded and edge points under every
 reaction / parameter variant, n = 4..20) or with the CPU oracle, which is
 bit-exact with the reference, beyond them (n = 16 and 32 for every
-variant).  FP64 bar per output: |gpu - ref| <= 1e-12 |ref| + 64 u m, m the
+variant).  FP64 bar per output: |gpu - ref| <= 1e-12 |ref| + 2 u m, m the
 running error magnitude of that output (oracle/error_bound.cpp, u = 2^-53):
 the plain 1e-12 relative bar where the output is well conditioned, the bound
 every double evaluation obeys where it is formed by cancellation.

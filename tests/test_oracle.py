@@ -57,7 +57,7 @@ def test_gauss_legendre_exactness():
     # Newton stops at |dz| <= 1e-10 (integration.hpp:30) and the weights use
     # the derivative at the previous iterate, so they carry errors up to ~1e-10
     # while the nodes are converged to rounding.
-    for n in (1, 4, 10, 16, 24):
+    for n in (1, 4, 10, 16, 24, 32):
         x, w = O.gauss_legendre(n)
         xr, wr = np.polynomial.legendre.leggauss(n)
         assert np.max(np.abs(x - 0.5 * (xr + 1))) < 1e-14
@@ -285,13 +285,16 @@ def test_m1_variants_match_reference(golden):
     assert same(out, g["out_equilibrium_n10"])
 
 
-def test_m1_kirchhoff_at_equilibrium():
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_m1_kirchhoff_at_equilibrium(nq):
     """With equilibrium Fermi-Dirac distributions and blocking on, every
     emission/absorption pair balances cell by cell (detailed balance), so
     eta_0 = c kappa_0_a n and eta = c kappa_a J per species.  Points whose
     |eta_nu| is large are excluded: there the reference's blocking factor
     1 - g cancels (g -> 1) and SafeExp's +-700 clamp engages
-    (functions.hpp:66-75), both of which break the balance numerically."""
+    (functions.hpp:66-75), both of which break the balance numerically.
+    n = 16 and 32 are beyond the reference's BS_N_MAX (bns_nurates.hpp:68):
+    the balance pins the oracle there."""
     pts = W.make_points(400, seed=5)
     eta_nu = np.abs(pts[W.P_MUE] - pts[W.P_MUN] + pts[W.P_MUP]) / pts[W.P_T]
     pts = np.ascontiguousarray(pts[:, eta_nu < 10.0])
@@ -301,7 +304,7 @@ def test_m1_kirchhoff_at_equilibrium():
     prm = np.array([0, 0, 1, 1, 1, 1, 1, 0], np.int32)
     for brem in (0, 1):
         prm[5] = brem
-        out = O.m1_opacities(meq, 8, W.FLAGS_ALL, prm, distr=deq)
+        out = O.m1_opacities(meq, nq, W.FLAGS_ALL, prm, distr=deq)
         for s in range(4):
             n, J = meq[W.P_N0 + s], meq[W.P_J0 + s]
             ok = (n > 1e-21) & (J > 1e-25) & (out[0 + s] > 1e-250)
@@ -310,15 +313,19 @@ def test_m1_kirchhoff_at_equilibrium():
             assert np.allclose(c * out[12 + s][ok] * J[ok], out[8 + s][ok], rtol=1e-9)
 
 
-def test_m1_additive_over_reactions():
+@pytest.mark.parametrize("nq", [6, 16, 32])
+def test_m1_additive_over_reactions(nq):
+    """The all-reactions outputs are the sum of the single-reaction ones
+    (the assembly adds the reactions' integrals, m1_opacities.hpp:1187-1292),
+    at the reference's orders and beyond its BS_N_MAX."""
     pts = W.make_points(32, seed=6)
     prm = np.array([1, 1, 1, 1, 1, 1, 1, 0], np.int32)
-    tot = O.m1_opacities(pts, 6, W.FLAGS_ALL, prm)
+    tot = O.m1_opacities(pts, nq, W.FLAGS_ALL, prm)
     acc = np.zeros_like(tot)
     for i in range(5):
         f = np.zeros(5, np.int32)
         f[i] = 1
-        acc += O.m1_opacities(pts, 6, f, prm)
+        acc += O.m1_opacities(pts, nq, f, prm)
     scale = np.maximum(np.abs(tot), 1e-300)
     big = np.abs(tot) > 1e-200
     assert np.max(np.abs(acc - tot)[big] / scale[big]) < 1e-10
