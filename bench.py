@@ -133,6 +133,7 @@ SERVE_SIZES = [8 ** 3, 16 ** 3, 32 ** 3, 64 ** 3]
 SERVE_WEIGHTS = [0.4, 0.3, 0.2, 0.1]
 SERVE_SINGLE = 0.05
 SERVE_ORDERS = [8, 10, 16]
+SERVE_CHUNK = 32768  # a 64^3 request runs as 8 pieces interleaved with the others
 
 
 def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
@@ -157,7 +158,8 @@ def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
         tpl.append(dict(n=n, pts=a, out=o, nquad=int(rng.choice(SERVE_ORDERS)),
                         precision=int(rng.integers(0, 2))))
     prm = W.params_default_all(W.BREM_BRT06)
-    srv = api.Server(0, hbm_budget_bytes=4 << 30, streams=4, max_batch_points=1 << 20)
+    srv = api.Server(0, hbm_budget_bytes=4 << 30, streams=4, max_batch_points=1 << 20,
+                     chunk_points=SERVE_CHUNK)
 
     def submit(t):
         return srv.submit(t["pts"].array, t["out"].array, t["nquad"], W.FLAGS_ALL, prm,
@@ -193,7 +195,7 @@ def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
                     if not inflight and stop[0]:
                         return
                     rid, k = inflight.popleft()
-                lat.append(srv.wait(rid))
+                lat.append((srv.wait(rid), tpl[k]["n"]))
                 with lock:
                     served[0] += tpl[k]["n"]
                     free.append(k)
@@ -222,11 +224,21 @@ def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
             lock.notify_all()
         th.join()
         wall = time.perf_counter() - t_start
-        lat_ms = np.array(lat) * 1e3
+        lat_ms = np.array([x for x, _ in lat]) * 1e3
+        size = np.array([n for _, n in lat])
         nb = srv.stats()["batches"] - b0
+        per_class = {}
+        for name, lo, hi in (("single", 1, 1), ("<=16^3", 2, 16 ** 3), ("32^3", 16 ** 3 + 1, 32 ** 3),
+                             ("64^3", 32 ** 3 + 1, 64 ** 3)):
+            sel = (size >= lo) & (size <= hi)
+            if sel.any():
+                per_class[name] = {"requests": int(sel.sum()),
+                                   "p50_ms": float(f"{np.percentile(lat_ms[sel], 50):.3g}"),
+                                   "p99_ms": float(f"{np.percentile(lat_ms[sel], 99):.3g}")}
         rows.append({"arrival_rate_rps": round(rate, 1), "requests": n_sub,
                      "p50_ms": float(f"{np.percentile(lat_ms, 50):.3g}"),
                      "p99_ms": float(f"{np.percentile(lat_ms, 99):.3g}"),
+                     "by_size": per_class,
                      "points_per_s": float(f"{served[0] / wall:.4g}"),
                      "requests_per_batch": float(f"{n_sub / max(nb, 1):.3g}")})
         log(f"serving at {rate:.0f} req/s: {rows[-1]}")
@@ -237,7 +249,9 @@ def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
         t["out"].close()
     return {"workload": "Poisson arrivals; meshblocks 8^3/16^3/32^3/64^3 (0.4/0.3/0.2/0.1) "
                         "+ 5% single points; n in {8,10,16}; FP64 or mixed per request; "
-                        "host inputs in pinned memory (PCIe copies included in latency)",
+                        "host inputs in pinned memory (PCIe copies included in latency); "
+                        f"requests above {SERVE_CHUNK} points as interleaved pieces, smallest "
+                        "request first",
             "capacity_points_per_s": float(f"{cap_pts:.4g}"), "sweep": rows,
             "server": {"streams": 4, "hbm_budget_bytes": st["arena_bytes"],
                        "max_inflight_bytes": st["max_inflight_bytes"],

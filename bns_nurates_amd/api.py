@@ -123,7 +123,8 @@ def as_params(params) -> OpacityParams:
 
 class _ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget_bytes", C.c_size_t), ("streams", C.c_int),
-                ("max_batch_points", C.c_int64)]
+                ("max_batch_points", C.c_int64), ("chunk_points", C.c_int64),
+                ("starvation_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
@@ -496,12 +497,16 @@ class PinnedBuffer:
 class Server:
     """Continuous-batching request server (bnsamd_server_*): submit host SoA
     point blocks [21, npts] from any thread, wait for their [20, npts]
-    outputs by id."""
+    outputs by id.  Requests above chunk_points points run as pieces that
+    interleave with other requests; the smallest request is served first
+    unless a piece has waited longer than starvation_ms."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, streams: int = 4,
-                 max_batch_points: int = 1 << 20):
+                 max_batch_points: int = 1 << 20, chunk_points: int = 32768,
+                 starvation_ms: float = 200.0):
         self.lib = load_library()
-        cfg = _ServerConfig(int(device), int(hbm_budget_bytes), int(streams), int(max_batch_points))
+        cfg = _ServerConfig(int(device), int(hbm_budget_bytes), int(streams), int(max_batch_points),
+                            int(chunk_points), float(starvation_ms))
         h = C.c_void_p()
         rc = self.lib.bnsamd_server_create(C.byref(cfg), C.byref(h))
         if rc:

@@ -8,11 +8,17 @@
 //              device arena allocated once at start-up (the HBM budget); a
 //              batch is admitted when the arena can hold it and one of the
 //              server's streams is free, so nothing is allocated per request
-//   batching   the pending request at the head of the queue opens a batch;
-//              later pending requests with the same quadrature order,
-//              precision, flags and parameters join it up to the batch size
-//              (points are independent, so a request's results do not depend
-//              on its batch);
+//   pieces     a request larger than the chunk size is split into pieces of
+//              at most that many points, queued one after another; the
+//              request completes with its last piece;
+//   batching   the pending piece of the smallest request (ties: the earliest)
+//              opens a batch, so that small requests do not wait behind a
+//              large one's pieces; a piece that has waited longer than the
+//              starvation limit goes first; later pending pieces with the
+//              same quadrature order, precision, flags and parameters join
+//              it up to the chunk size, so that no launch holds the device
+//              for longer than one chunk (points are independent, so a
+//              request's results do not depend on its batch or pieces);
 //   overlap    batches run on separate HIP streams (upload, kernel, download,
 //              event); requests that arrive while others are on the device
 //              are admitted as soon as a stream and arena space free up.
@@ -111,6 +117,7 @@ class Arena {
 
 struct Request {
   int64_t id = 0, npts = 0;
+  int64_t pieces_left = 0;
   const double* h_pts = nullptr;
   double* h_out = nullptr;
   bool pinned_in = false, pinned_out = false;  // page-locked caller buffers
@@ -133,6 +140,13 @@ Key key_of(const Request& r) {
              p.neglect_blocking};
 }
 
+// rows [off, off + n) of a request's point and output blocks
+struct Piece {
+  Request* r = nullptr;
+  int64_t off = 0, n = 0;
+  Clock::time_point t_queued;
+};
+
 // whether host memory is page-locked (copies from it run asynchronously)
 bool is_pinned(const void* p) {
   hipPointerAttribute_t a;
@@ -152,7 +166,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
-  std::vector<Request*> reqs;
+  std::vector<Piece> reqs;
   size_t off = 0, bytes = 0;
   int64_t npts = 0;
 };
@@ -169,10 +183,12 @@ struct bnsamd_server {
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Request*> pending;
+  std::deque<Piece> pending;
   std::map<int64_t, std::unique_ptr<Request>> requests;
   int64_t next_id = 1;
   bool stop = false;
+  int64_t chunk = 0;                    // points per piece
+  Clock::duration starve{};             // a piece waiting longer goes first
   std::string err;
   bnsamd_server_stats stats{};
   std::thread worker;
@@ -202,12 +218,15 @@ struct bnsamd_server {
 
   void finish(Slot& s, int status) {
     Clock::time_point now = Clock::now();
-    for (Request* r : s.reqs) {
-      r->done = true;
-      r->status = status;
-      r->t_done = now;
-      stats.requests_completed += 1;
-      stats.points_completed += r->npts;
+    for (const Piece& pc : s.reqs) {
+      Request* r = pc.r;
+      if (status != BNSAMD_OK) r->status = status;
+      stats.points_completed += pc.n;
+      if (--r->pieces_left == 0) {
+        r->done = true;
+        r->t_done = now;
+        stats.requests_completed += 1;
+      }
     }
     arena.free(s.off, s.bytes);
     s.reqs.clear();
@@ -237,29 +256,46 @@ struct bnsamd_server {
           break;
         }
       if (!slot) return;
-      // the head request opens the batch; same-key requests behind it join
-      Request* head = pending.front();
-      const Key k = key_of(*head);
-      std::vector<Request*> take{head};
-      int64_t n = head->npts;
-      for (size_t i = 1; i < pending.size(); ++i) {
-        Request* r = pending[i];
-        if (n + r->npts > cfg.max_batch_points) continue;
-        if (key_of(*r) == k) {
-          take.push_back(r);
-          n += r->npts;
+      // pieces in the order they are served: a piece waiting beyond the
+      // starvation limit first (oldest first), then by the size of its
+      // request (smallest first), then by queue order (pieces of one request
+      // keep theirs)
+      const Clock::time_point now = Clock::now();
+      std::vector<size_t> order(pending.size());
+      for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+      auto starved = [&](const Piece& pc) { return now - pc.t_queued > starve; };
+      std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const Piece &pa = pending[a], &pb = pending[b];
+        const bool sa = starved(pa), sb = starved(pb);
+        if (sa != sb) return sa;
+        if (sa) return false;  // both starved: queue order
+        return pa.r->npts < pb.r->npts;
+      });
+      // the first piece opens the batch; same-key pieces in that order join
+      const Key k = key_of(*pending[order[0]].r);
+      std::vector<size_t> take_idx{order[0]};
+      int64_t n = pending[order[0]].n;
+      for (size_t j = 1; j < order.size(); ++j) {
+        const Piece& pc = pending[order[j]];
+        if (n + pc.n > chunk) continue;  // a launch covers at most one chunk of points
+        if (key_of(*pc.r) == k) {
+          take_idx.push_back(order[j]);
+          n += pc.n;
         }
       }
       size_t bytes = batch_bytes(n);
       int64_t off = arena.alloc(bytes);
-      while (off < 0 && take.size() > 1) {  // shrink to what fits now
-        n -= take.back()->npts;
-        take.pop_back();
+      while (off < 0 && take_idx.size() > 1) {  // shrink to what fits now
+        n -= pending[take_idx.back()].n;
+        take_idx.pop_back();
         bytes = batch_bytes(n);
         off = arena.alloc(bytes);
       }
       if (off < 0) return;  // admission: wait for in-flight batches to free memory
-      for (Request* r : take) pending.erase(std::find(pending.begin(), pending.end(), r));
+      std::vector<Piece> take;
+      for (size_t i : take_idx) take.push_back(pending[i]);
+      std::sort(take_idx.begin(), take_idx.end());
+      for (size_t j = take_idx.size(); j-- > 0;) pending.erase(pending.begin() + take_idx[j]);
       slot->busy = true;
       slot->reqs = std::move(take);
       slot->off = size_t(off);
@@ -286,33 +322,34 @@ struct bnsamd_server {
   // (lock free) upload, kernel and download of one batch on its stream
   int enqueue(Slot& s) {
     const Tables* tab = nullptr;
-    const Request& h = *s.reqs.front();
+    const Request& h = *s.reqs.front().r;
     if (int rc = table_for(h.nquad, &tab)) return rc;
     const int64_t B = s.npts;
     const size_t ld = size_t(B) * sizeof(double);
     double* d_pts = reinterpret_cast<double*>(arena.base() + s.off);
     double* d_out = d_pts + size_t(B) * BNSAMD_PT_NFIELDS;
     bool staged = false;
-    for (Request* r : s.reqs) staged = staged || !r->pinned_in || !r->pinned_out;
+    for (const Piece& pc : s.reqs) staged = staged || !pc.r->pinned_in || !pc.r->pinned_out;
     if (staged)
       if (int rc = ensure_mirror()) return rc;
     double* m_pts = staged ? reinterpret_cast<double*>(hmirror + s.off) : nullptr;
     double* m_out = staged ? m_pts + size_t(B) * BNSAMD_PT_NFIELDS : nullptr;
     int64_t o = 0;
-    for (Request* r : s.reqs) {
-      const size_t w = size_t(r->npts) * sizeof(double);
-      const double* src = r->h_pts;
-      size_t spitch = w;
+    for (const Piece& pc : s.reqs) {
+      const Request* r = pc.r;
+      const size_t w = size_t(pc.n) * sizeof(double);
+      const double* src = r->h_pts + pc.off;
+      size_t spitch = size_t(r->npts) * sizeof(double);
       if (!r->pinned_in) {  // stage into the pinned mirror, then copy from there
         for (int f = 0; f < BNSAMD_PT_NFIELDS; ++f)
-          std::memcpy(m_pts + size_t(f) * B + o, r->h_pts + size_t(f) * r->npts, w);
+          std::memcpy(m_pts + size_t(f) * B + o, r->h_pts + size_t(f) * r->npts + pc.off, w);
         src = m_pts + o;
         spitch = ld;
       }
       if (hipMemcpy2DAsync(d_pts + o, ld, src, spitch, w, BNSAMD_PT_NFIELDS,
                            hipMemcpyHostToDevice, s.stream) != hipSuccess)
         return BNSAMD_ERR_HIP;
-      o += r->npts;
+      o += pc.n;
     }
     bns::GrayKernelArgs a{};
     a.npts = B;
@@ -334,14 +371,15 @@ struct bnsamd_server {
                                                : bns::launch_gray_m1_f64(a, s.stream);
     if (e != hipSuccess) return BNSAMD_ERR_HIP;
     o = 0;
-    for (Request* r : s.reqs) {
-      const size_t w = size_t(r->npts) * sizeof(double);
-      double* dst = r->pinned_out ? r->h_out : m_out + o;
-      size_t dpitch = r->pinned_out ? w : ld;
+    for (const Piece& pc : s.reqs) {
+      const Request* r = pc.r;
+      const size_t w = size_t(pc.n) * sizeof(double);
+      double* dst = r->pinned_out ? r->h_out + pc.off : m_out + o;
+      size_t dpitch = r->pinned_out ? size_t(r->npts) * sizeof(double) : ld;
       if (hipMemcpy2DAsync(dst, dpitch, d_out + o, ld, w, BNSAMD_OUT_NFIELDS,
                            hipMemcpyDeviceToHost, s.stream) != hipSuccess)
         return BNSAMD_ERR_HIP;
-      o += r->npts;
+      o += pc.n;
     }
     return hipEventRecord(s.done, s.stream) == hipSuccess ? BNSAMD_OK : BNSAMD_ERR_HIP;
   }
@@ -353,12 +391,13 @@ struct bnsamd_server {
         hmirror ? reinterpret_cast<double*>(hmirror + s.off) + size_t(B) * BNSAMD_PT_NFIELDS
                 : nullptr;
     int64_t o = 0;
-    for (Request* r : s.reqs) {
+    for (const Piece& pc : s.reqs) {
+      const Request* r = pc.r;
       if (!r->pinned_out && m_out)
         for (int f = 0; f < BNSAMD_OUT_NFIELDS; ++f)
-          std::memcpy(r->h_out + size_t(f) * r->npts, m_out + size_t(f) * B + o,
-                      size_t(r->npts) * sizeof(double));
-      o += r->npts;
+          std::memcpy(r->h_out + size_t(f) * r->npts + pc.off, m_out + size_t(f) * B + o,
+                      size_t(pc.n) * sizeof(double));
+      o += pc.n;
     }
   }
 
@@ -412,7 +451,7 @@ int bnsamd_server_create(const bnsamd_server_config* cfg, bnsamd_server** out) {
   if (!out || !cfg) return BNSAMD_ERR_INVALID;
   *out = nullptr;
   if (cfg->streams < 1 || cfg->streams > 16 || cfg->max_batch_points < 1 ||
-      cfg->hbm_budget_bytes < batch_bytes(1))
+      cfg->hbm_budget_bytes < batch_bytes(1) || cfg->chunk_points < 0 || !(cfg->starvation_ms >= 0.))
     return BNSAMD_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BNSAMD_ERR_NO_DEVICE;
@@ -420,6 +459,10 @@ int bnsamd_server_create(const bnsamd_server_config* cfg, bnsamd_server** out) {
   if (hipSetDevice(cfg->device) != hipSuccess) return BNSAMD_ERR_HIP;
   auto s = std::make_unique<bnsamd_server>();
   s->cfg = *cfg;
+  s->chunk = cfg->chunk_points > 0 ? cfg->chunk_points : 32768;
+  if (s->chunk > cfg->max_batch_points) s->chunk = cfg->max_batch_points;
+  s->starve = std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double, std::milli>(
+      cfg->starvation_ms > 0. ? cfg->starvation_ms : 200.));
   s->gamma36 = bns::gamma_stirling_36();
   if (s->arena.init(cfg->hbm_budget_bytes) != hipSuccess) return BNSAMD_ERR_NOMEM;
   s->stats.arena_bytes = int64_t(s->arena.size());
@@ -478,8 +521,8 @@ int bnsamd_server_submit(bnsamd_server* s, int64_t npts, const double* h_pts, in
   if (stim_abs != 0 && stim_abs != 1) return bad("submit: stim_abs must be 0 or 1");
   if (params->brem_implementation < 0 || params->brem_implementation > 2)
     return bad("submit: unknown brem_implementation");
-  if (npts > int64_t(0x7fffffff) || batch_bytes(npts) > s->arena.size())
-    return bad("submit: request larger than the server's HBM budget");
+  if (npts > int64_t(0x7fffffff) || batch_bytes(std::min(npts, s->chunk)) > s->arena.size())
+    return bad("submit: request piece larger than the server's HBM budget");
   if (s->stop) return bad("submit: server is stopping");
   auto r = std::make_unique<Request>();
   r->id = s->next_id++;
@@ -495,7 +538,15 @@ int bnsamd_server_submit(bnsamd_server* s, int64_t npts, const double* h_pts, in
   r->params = *params;
   r->t_submit = Clock::now();
   *request_id = r->id;
-  s->pending.push_back(r.get());
+  for (int64_t off = 0; off < npts; off += s->chunk) {
+    Piece pc;
+    pc.r = r.get();
+    pc.off = off;
+    pc.n = std::min(s->chunk, npts - off);
+    pc.t_queued = r->t_submit;
+    s->pending.push_back(pc);
+    r->pieces_left += 1;
+  }
   s->requests.emplace(r->id, std::move(r));
   s->stats.requests_submitted += 1;
   s->cv_work.notify_one();

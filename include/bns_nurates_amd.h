@@ -209,10 +209,12 @@ int bnsamd_fdi(bnsamd_context* ctx, int k, int64_t n, const double* d_y, double*
 
 /* ---- serving --------------------------------------------------------------
  * A request server for many concurrent callers (meshblocks of uneven sizes):
- * requests are queued, admitted against a fixed device-memory budget,
- * batched with other pending requests of the same quadrature order,
- * precision, flags and parameters, and run on several streams so that their
- * work overlaps.  A request's results do not depend on its batch.  No
+ * requests are queued (a large one as pieces of chunk_points points),
+ * admitted against a fixed device-memory budget, served smallest request
+ * first (a piece waiting beyond starvation_ms goes first), batched with other
+ * pending pieces of the same quadrature order, precision, flags and
+ * parameters, and run on several streams so that their work overlaps.  A
+ * request's results do not depend on its batch or pieces.  No
  * counterpart in the reference, whose callers invoke ComputeM1Opacities per
  * point inside their own loops (tests/tests.hpp:272-274,756); each request is
  * the batched equivalent of such a loop, with the distribution reconstructed
@@ -225,7 +227,10 @@ typedef struct bnsamd_server_config {
   int device;               /* HIP device */
   size_t hbm_budget_bytes;  /* device arena for in-flight batches: 328 B per point */
   int streams;              /* batches in flight at once, 1..16 */
-  int64_t max_batch_points; /* points per launch (a larger single request runs alone) */
+  int64_t max_batch_points; /* upper limit of chunk_points */
+  int64_t chunk_points;     /* points per launch: a larger request runs as pieces of this
+                               many points, interleaved with other requests (0: 32768) */
+  double starvation_ms;     /* a piece queued longer than this is served first (0: 200) */
 } bnsamd_server_config;
 
 typedef struct bnsamd_server_stats {
@@ -242,7 +247,7 @@ const char* bnsamd_server_last_error(const bnsamd_server* s);
  * h_out[20][npts] are written when the request completes.  Both buffers must
  * stay valid until bnsamd_server_wait returns for this request; pinned
  * buffers (bnsamd_host_alloc) let the copies run asynchronously.  Rejects a
- * request larger than the server's HBM budget. */
+ * request whose piece is larger than the server's HBM budget. */
 int bnsamd_server_submit(bnsamd_server* s, int64_t npts, const double* h_pts, int nquad,
                          const bnsamd_flags* flags, const bnsamd_params* params, int stim_abs,
                          int precision, double* h_out, int64_t* request_id);
