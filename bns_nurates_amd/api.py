@@ -167,6 +167,8 @@ def load_library():
         "bnsamd_synchronize": ([vp], C.c_int),
         "bnsamd_gauss_legendre": ([C.c_int, C.c_double, C.c_double, dp, dp], C.c_int),
         "bnsamd_set_quadrature": ([vp, C.c_int, dp, dp, C.c_int, dp, dp], C.c_int),
+        "bnsamd_set_gp19_table": ([vp, C.c_int, dp, dp, C.c_int, dp, C.c_int, dp, C.c_int, dp,
+                                   dp, dp], C.c_int),
         "bnsamd_m1_opacities": ([vp, i64, i64, vp, vp, C.POINTER(_Flags), C.POINTER(_Params),
                                  C.c_int, C.c_int, vp], C.c_int),
         "bnsamd_m1_opacities_host": ([vp, i64, vp, vp, C.POINTER(_Flags), C.POINTER(_Params),
@@ -315,6 +317,24 @@ class Context:
                                                    len(x2), _dptr(x2), _dptr(w2)),
                     "bnsamd_set_quadrature")
         self._quad = (len(x1), len(x2))
+
+    def set_gp19_table(self, table):
+        """Installs the GP19 bremsstrahlung table (a dict with the axes nb
+        [fm^-3], lognb, ye, T [MeV], w [MeV], logw and data [nnb, nye, nT,
+        nw]; bnsamd_set_gp19_table) or clears it (None)."""
+        if table is None:
+            self._check(self.lib.bnsamd_set_gp19_table(self.h, 0, None, None, 0, None, 0, None, 0,
+                                                       None, None, None), "bnsamd_set_gp19_table")
+            return
+        ax = {k: np.ascontiguousarray(table[k], dtype=np.float64)
+              for k in ("nb", "lognb", "ye", "T", "w", "logw")}
+        data = np.ascontiguousarray(table["data"], dtype=np.float64)
+        if data.shape != (len(ax["nb"]), len(ax["ye"]), len(ax["T"]), len(ax["w"])):
+            raise BnsAmdError("gp19 table: data must be [nnb, nye, nT, nw]")
+        self._check(self.lib.bnsamd_set_gp19_table(
+            self.h, len(ax["nb"]), _dptr(ax["nb"]), _dptr(ax["lognb"]), len(ax["ye"]),
+            _dptr(ax["ye"]), len(ax["T"]), _dptr(ax["T"]), len(ax["w"]), _dptr(ax["w"]),
+            _dptr(ax["logw"]), _dptr(data)), "bnsamd_set_gp19_table")
 
     def compute_m1_opacities(self, pts, flags=None, params=None, stim_abs: int = 1,
                              precision: int = FP64, distr=None, out=None, npts=None):

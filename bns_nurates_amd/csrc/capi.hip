@@ -70,6 +70,11 @@ struct bnsamd_context {
   size_t stage_doubles = 0;
   // this device's static staging block (bnsamd_m1_opacities_host_static)
   double* scratch = nullptr;
+  // GP19 bremsstrahlung table (bnsamd_set_gp19_table): one device block, the
+  // bns::Gp19Table that describes it first, then axes and data; gp19 points
+  // to that struct (nullptr: no table)
+  double* gp19_dev = nullptr;
+  const bns::Gp19Table* gp19 = nullptr;
 };
 
 namespace {
@@ -240,6 +245,7 @@ int run_gray(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pts,
                               params->neglect_blocking};
   a.stim = stim_abs;
   a.gamma36 = bns::gamma_stirling_36();
+  a.gp19 = c->gp19;
   hipError_t e;
   if (precision == BNSAMD_FP64) {
     e = bns::launch_gray_m1_f64(a, c->stream);
@@ -290,6 +296,7 @@ int run_spectral(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pt
                               params->neglect_blocking};
   a.stim = stim_abs;
   a.gamma36 = bns::gamma_stirling_36();
+  a.gp19 = c->gp19;
   hipError_t e = bns::launch_spectral_f64(a, c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "spectral kernel launch");
   return BNSAMD_OK;
@@ -328,6 +335,7 @@ void bnsamd_destroy(bnsamd_context* c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->cache) (void)hipFree(kv.second.dev);
   if (c->stage) (void)hipFree(c->stage);
+  if (c->gp19_dev) (void)hipFree(c->gp19_dev);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
@@ -427,6 +435,59 @@ int bnsamd_set_quadrature(bnsamd_context* c, int n1, const double* x1, const dou
   }
   auto ins = c->cache.emplace(std::move(key), t);
   c->current = &ins.first->second;
+  return BNSAMD_OK;
+}
+
+int bnsamd_set_gp19_table(bnsamd_context* c, int n_nb, const double* nb, const double* lognb,
+                          int n_ye, const double* ye, int n_T, const double* T, int n_w,
+                          const double* w, const double* logw, const double* data) {
+  if (check_ctx(c)) return BNSAMD_ERR_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  // the previous table may be read by work still queued on the stream
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "bnsamd_set_gp19_table: stream");
+  if (c->gp19_dev) (void)hipFree(c->gp19_dev);
+  c->gp19_dev = nullptr;
+  c->gp19 = nullptr;
+  if (n_nb == 0) return BNSAMD_OK;  // cleared: GP19 adds zero again
+  auto increasing = [](int n, const double* a) {
+    if (!a || n < 2) return false;
+    for (int i = 0; i < n; ++i)
+      if (!std::isfinite(a[i]) || (i && !(a[i] > a[i - 1]))) return false;
+    return true;
+  };
+  if (!increasing(n_nb, nb) || !increasing(n_nb, lognb) || !increasing(n_ye, ye) ||
+      !increasing(n_T, T) || !increasing(n_w, w) || !increasing(n_w, logw) || !data)
+    return fail(c, BNSAMD_ERR_INVALID,
+                "bnsamd_set_gp19_table: axes need >= 2 finite, strictly increasing values");
+  const size_t nd = size_t(n_nb) * n_ye * n_T * n_w;
+  constexpr size_t head = (sizeof(bns::Gp19Table) + sizeof(double) - 1) / sizeof(double);
+  const size_t total = head + 2 * size_t(n_nb) + n_ye + n_T + 2 * size_t(n_w) + nd;
+  std::vector<double> host(total);
+  double* h = host.data();
+  const double* parts[7] = {nb, lognb, ye, T, w, logw, data};
+  const size_t sizes[7] = {size_t(n_nb), size_t(n_nb), size_t(n_ye), size_t(n_T), size_t(n_w),
+                           size_t(n_w), nd};
+  size_t off[7];
+  size_t o = head;
+  for (int k = 0; k < 7; o += sizes[k], ++k) {
+    off[k] = o;
+    std::memcpy(h + o, parts[k], sizes[k] * sizeof(double));
+  }
+  double* d = nullptr;
+  if (hipMalloc(&d, total * sizeof(double)) != hipSuccess)
+    return fail(c, BNSAMD_ERR_NOMEM, "bnsamd_set_gp19_table: hipMalloc");
+  const bns::Gp19Table tab{n_nb, n_ye, n_T, n_w, d + off[0], d + off[1], d + off[2], d + off[3],
+                           d + off[4], d + off[5], d + off[6]};
+  std::memcpy(h, &tab, sizeof(tab));
+  e = hipMemcpy(d, h, total * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(c, e, "bnsamd_set_gp19_table: upload");
+  }
+  c->gp19_dev = d;
+  c->gp19 = reinterpret_cast<const bns::Gp19Table*>(d);
   return BNSAMD_OK;
 }
 

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "gp19.h"
 #include "gray_tables.h"
 
 namespace bns {
@@ -30,6 +31,7 @@ struct GrayKernelArgs {
   PointParams params;
   int stim;                   // 1: ComputeM1Opacities, 0: ...NotStimulated
   double gamma36;             // GammaStirling(3.6) (functions.hpp:4460-4470)
+  const Gp19Table* gp19;      // GP19 bremsstrahlung table in device memory, or nullptr
 };
 
 hipError_t launch_gray_m1_f64(const GrayKernelArgs& a, hipStream_t stream);

@@ -16,6 +16,7 @@
 
 #include <cstdint>
 
+#include "gp19.h"
 #include "gray_kernel.h"
 #include "gray_math.h"
 #include "point_physics.h"
@@ -34,7 +35,9 @@ constexpr int kEtaSlot = 61;
 // fixed per launch, so that a cell evaluates only its own weighting.
 // NQ > 0: both quadrature orders fixed at NQ at compile time (table
 // offsets and loop bounds are constants); 0: taken from the arguments.
-template <bool MIX, bool STIM, bool BLOCK, int NQ>
+// GP19: the instance for GP19 bremsstrahlung over a table (its interpolation
+// and extrapolations kept out of the other instances' register budget).
+template <bool MIX, bool STIM, bool BLOCK, int NQ, bool GP19 = false>
 __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_Fm[5][64];  // F_1..F_5 (eta - y_a)
   __shared__ double s_Fp[5][64];  // F_1..F_5 (eta + y_a)
@@ -217,7 +220,9 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
 
   // ========================= 2D: pair + bremsstrahlung =======================
   const bool do_pair = A.flags.use_pair == 1;
-  const bool do_brem = A.flags.use_brem == 1 && A.params.brem_impl != 2;
+  // GP19 without a table adds zero, as the reference's table-free
+  // configuration does (kernel_brem_GP19.hpp:483-493)
+  const bool do_brem = A.flags.use_brem == 1 && (A.params.brem_impl != 2 || GP19);
   const int N = L.N;
   if (A.flags.use_pair == 1 || A.flags.use_brem == 1) {
     // node tables.  The node energy nu_a = t x_a or t / x_a with t = 2.182 T
@@ -281,7 +286,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
                  (rho * rho) * exp(5.5 * log(T));
       double T2 = T * T, T7 = T2 * T2 * T2 * T;
       brt_pre = c::HC6FourPiSq * c::BremC4BRT06 * (q / T7);
-    } else if (do_brem) {
+    } else if (do_brem && A.params.brem_impl == 0) {
       // BremKernelS / BremKernelG / single channels, the parts that do not
       // depend on x (kernel_brem_HR98.hpp:33-338)
       const HrPoint hp = hr98_point(T, vin[0], A.params.use_NN_medium_corr != 0);
@@ -372,7 +377,11 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       if (do_brem) {
         const double brt_pre = vbrem[0], hr_y = vbrem[1], hr_emy = vbrem[2];
         const double hr_p = vbrem[3], hr_pinv = vbrem[4], hr_medium = vbrem[5];
-        if (A.params.brem_impl == 1) {
+        if constexpr (GP19) {
+          // BremKernelAbsGP19 at omega = nu_a, omega' = nu_b (m1_opacities.hpp:433-470),
+          // the electron fraction Ye taken as yp (the point block has no ye)
+          gp19_brem(*A.gp19, node_nu(a), node_nu(b), vin[0], vin[2], vin[1], bem, bab);
+        } else if (A.params.brem_impl == 1) {
           bem = brt_pre * cr[CL_K1EM];
           bab = brt_pre * cr[CL_K1AB];
         } else if constexpr (MIX) {
@@ -707,18 +716,18 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   if (lane < 20) A.out[lane * ld + p] = vin[lane];
 }
 
-template <int WAVES_PER_EU, bool STIM, bool BLOCK, int NQ>
+template <int WAVES_PER_EU, bool STIM, bool BLOCK, int NQ, bool GP19 = false>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_f64(
     GrayKernelArgs A) {
-  gray_point<false, STIM, BLOCK, NQ>(A);
+  gray_point<false, STIM, BLOCK, NQ, GP19>(A);
 }
 
-template <int WAVES_PER_EU, bool STIM, bool BLOCK, int NQ>
+template <int WAVES_PER_EU, bool STIM, bool BLOCK, int NQ, bool GP19 = false>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU, WAVES_PER_EU))) void k_gray_m1_mixed(
     GrayKernelArgs A) {
-  gray_point<true, STIM, BLOCK, NQ>(A);
+  gray_point<true, STIM, BLOCK, NQ, GP19>(A);
 }
 
 }  // namespace bns
@@ -734,6 +743,14 @@ void launch_instance(const GrayKernelArgs& a, dim3 grid, hipStream_t stream) {
   // the 128^3-grid workload's order, n = 16, has its own instance (0.6%
   // faster: its table offsets and loop bounds are constants)
   const bool fixed = a.L.n1 == 16 && a.L.n2 == 16;
+  // GP19 bremsstrahlung over a table: its own instances, 2 waves per SIMD
+  if (a.gp19 && a.params.brem_impl == 2 && a.flags.use_brem == 1) {
+    if constexpr (MIX)
+      hipLaunchKernelGGL((k_gray_m1_mixed<2, STIM, BLOCK, 0, true>), grid, dim3(64), 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_gray_m1_f64<2, STIM, BLOCK, 0, true>), grid, dim3(64), 0, stream, a);
+    return;
+  }
   if constexpr (MIX) {
     if (fixed)
       hipLaunchKernelGGL((k_gray_m1_mixed<W, STIM, BLOCK, 16>), grid, dim3(64), 0, stream, a);

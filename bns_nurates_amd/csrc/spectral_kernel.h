@@ -29,6 +29,7 @@ struct SpectralKernelArgs {
   PointParams params;
   int stim;              // 1: ComputeSpectralOpacitiesStimulatedAbs, 0: ...NotStimulatedAbs
   double gamma36;        // GammaStirling(3.6)
+  const Gp19Table* gp19; // GP19 bremsstrahlung table in device memory, or nullptr
 };
 
 // Energies per chunk for quadrature order n (a multiple of the energies one

@@ -122,7 +122,8 @@ __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
 // quadrature order and energies per chunk fixed at compile time (the LDS
 // offsets become constants instead of values held in registers), 0: taken
 // from the arguments.
-template <int WAVES, int NQ, int ECH>
+// GP19: the instance for GP19 bremsstrahlung over a table.
+template <int WAVES, int NQ, int ECH, bool GP19 = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void
 k_spectral_f64(SpectralKernelArgs A) {
   extern __shared__ double sm[];
@@ -166,7 +167,9 @@ k_spectral_f64(SpectralKernelArgs A) {
   if (lane == 63) s_in[kInvT] = 1. / T;
   const double eta = mu_e / T;
   const bool do_pair = A.flags.use_pair != 0;
-  const bool do_brem = A.flags.use_brem != 0 && A.params.brem_impl != 2;
+  // GP19 without a table adds zero, as the reference's table-free
+  // configuration does (kernel_brem_GP19.hpp:483-493)
+  const bool do_brem = A.flags.use_brem != 0 && (A.params.brem_impl != 2 || GP19);
   const bool do_pb = A.flags.use_pair != 0 || A.flags.use_brem != 0;
   const bool do_neps = A.flags.use_inelastic_scatt == 1;
   const bool blocking = A.params.neglect_blocking == 0;
@@ -199,7 +202,7 @@ k_spectral_f64(SpectralKernelArgs A) {
                (rho * rho) * exp(5.5 * log(T));
     double T2 = T * T, T7 = T2 * T2 * T2 * T;
     brt_pre = c::HC6FourPiSq * c::BremC4BRT06 * (q / T7);
-  } else if (do_brem) {
+  } else if (do_brem && A.params.brem_impl == 0) {
     hp = hr98_point(T, vin[0], A.params.use_NN_medium_corr != 0);
     const HrChan hc = hr98_channel(lane < 3 ? lane : 0, T, vin[0], vin[2], vin[3], hp.y);
     if (lane < 3) s_hc[lane] = hc;
@@ -434,7 +437,10 @@ k_spectral_f64(SpectralKernelArgs A) {
           }
           double se = 0., sa = 0.;
           if (do_brem) {
-            if (A.params.brem_impl == 1) {
+            if constexpr (GP19) {
+              // BremKernelAbsGP19 (m1_opacities.hpp:1386-1390), Ye taken as yp
+              gp19_brem(*A.gp19, nu, nub, vin[0], vin[2], T, se, sa);
+            } else if (A.params.brem_impl == 1) {
               // BremKernelsBRT06 at x = (w + w') / (2T) (kernel_brem_BRT06.hpp:42-69)
               const double xb = over_T(0.5 * (nu + nub));
               const double k1x = bessk1_fit(xb) / xb;
@@ -643,7 +649,9 @@ hipError_t launch_spectral_f64(const SpectralKernelArgs& a, hipStream_t stream) 
   const int64_t groups = a.npts * ((a.nchunk + a.cpw - 1) / a.cpw);
   dim3 grid(static_cast<unsigned>(groups)), block(kWave);
   const size_t lds = spectral_lds_bytes(a.n, a.ech);
-  if (a.n == kFixedN && a.ech == kFixedE)
+  if (a.gp19 && a.params.brem_impl == 2 && a.flags.use_brem != 0)  // GP19 over a table
+    hipLaunchKernelGGL((k_spectral_f64<2, 0, 0, true>), grid, block, lds, stream, a);
+  else if (a.n == kFixedN && a.ech == kFixedE)
     hipLaunchKernelGGL((k_spectral_f64<4, kFixedN, kFixedE>), grid, block, lds, stream, a);
   else
     hipLaunchKernelGGL((k_spectral_f64<3, 0, 0>), grid, block, lds, stream, a);

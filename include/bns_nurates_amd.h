@@ -72,8 +72,9 @@ typedef struct bnsamd_flags {
 } bnsamd_flags;
 
 /* OpacityParams (bns_nurates.hpp:338-352) with every bool as int;
- * brem_implementation: 0 BREM_HR98, 1 BREM_BRT06, 2 BREM_GP19 (adds zero, as
- * the reference's table-free configuration does, kernel_brem_GP19.hpp:483) */
+ * brem_implementation: 0 BREM_HR98, 1 BREM_BRT06, 2 BREM_GP19 (over the
+ * context's bnsamd_set_gp19_table; without one it adds zero, as the
+ * reference's table-free configuration does, kernel_brem_GP19.hpp:483) */
 typedef struct bnsamd_params {
   int use_dU, use_dm_eff, use_WM_ab, use_WM_sc, use_decay, brem_implementation,
       use_NN_medium_corr, neglect_blocking;
@@ -112,6 +113,23 @@ int bnsamd_gauss_legendre(int n, double x1, double x2, double* points, double* w
  * BS_N_MAX = 20 (bns_nurates.hpp:68). */
 int bnsamd_set_quadrature(bnsamd_context* ctx, int n1, const double* x1, const double* w1,
                           int n2, const double* x2, const double* w2);
+
+/* The GP19 bremsstrahlung table (BremKernelAbsGP19, kernel_brem_GP19.hpp:
+ * 35-480, whose GP19Table.hpp data the reference tree does not ship): the
+ * response function S(nb, Ye, T, w) on the grid of axes nb [fm^-3] (with its
+ * natural logarithms lognb, as the table states them), Ye, T [MeV] and the
+ * pair energy w [MeV] (with logw), data[((i n_ye + j) n_T + k) n_w + l]; each
+ * axis strictly increasing with at least 2 values.  Copied to the device;
+ * brem_implementation = BREM_GP19 then uses it in bnsamd_m1_opacities and
+ * bnsamd_spectral_opacities (the 2 and 4 MeV and 48 and 50 MeV temperatures
+ * of the extrapolations must lie on the T axis, as in the reference's
+ * table).  The electron fraction is the point block's yp (there is no
+ * separate ye field; GP19 is the only reader of ye on the path).  n_nb = 0
+ * clears the table: BREM_GP19 then adds zero, as the reference's table-free
+ * configuration does (kernel_brem_GP19.hpp:483-493). */
+int bnsamd_set_gp19_table(bnsamd_context* ctx, int n_nb, const double* nb, const double* lognb,
+                          int n_ye, const double* ye, int n_T, const double* T, int n_w,
+                          const double* w, const double* logw, const double* data);
 
 /* Gray M1 opacities of npts points, device buffers, asynchronous on the
  * context stream.  Replaces ComputeM1Opacities (stim_abs = 1) and

@@ -263,6 +263,15 @@ class Context {
                                 quad_2d->points, quad_2d->w));
   }
 
+  // The GP19 bremsstrahlung table that BREM_GP19 then interpolates
+  // (bnsamd_set_gp19_table; the reference's GP19Table.hpp arrays, which its
+  // tree does not ship); n_nb = 0 clears it (GP19 adds zero).
+  void SetGP19Table(int n_nb, const double* nb, const double* lognb, int n_ye, const double* ye,
+                    int n_T, const double* T, int n_w, const double* w, const double* logw,
+                    const double* data) {
+    check(bnsamd_set_gp19_table(ctx_, n_nb, nb, lognb, n_ye, ye, n_T, T, n_w, w, logw, data));
+  }
+
   // Device SoA batch, asynchronous on the context stream.
   void ComputeM1OpacitiesDevice(int64_t npts, int64_t ld, const double* d_pts,
                                 const double* d_distr, const OpacityFlags& flags,

@@ -34,6 +34,7 @@ _LIBS = {
     "ref_f32": os.path.join(HERE, "_ref", "libbnsref_f32.so"),
 }
 _cache = {}
+_gp19 = None  # the GP19 table installed in the oracle builds (set_gp19_table)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -66,6 +67,7 @@ def load(kind: str = "orc"):
                       C.c_int, _dp, _dp, C.c_int]
         f.restype = C.c_int
         _cache[kind] = (lib, p)
+        _apply_gp19(kind)
         return _cache[kind]
     if kind == "count":
         f = lib.orc_count_m1_opacities
@@ -98,8 +100,65 @@ def load(kind: str = "orc"):
         fn = getattr(lib, f"{p}_{name}")
         fn.argtypes = args
         fn.restype = res
+    if p == "orc":
+        for name, args in (("gp19_interp", [_i64, _dp, _dp, _dp, _dp, _dp]),
+                           ("brem_gp19", [_i64, _dp, _dp, _dp, _dp, _dp, _dp, _dp])):
+            fn = getattr(lib, f"orc_{name}")
+            fn.argtypes = args
+            fn.restype = None
     _cache[kind] = (lib, p)
+    _apply_gp19(kind)
     return _cache[kind]
+
+
+_GP19_KEYS = ("nb", "lognb", "ye", "T", "w", "logw")
+
+
+def _apply_gp19(kind):
+    """Installs the current GP19 table in one loaded oracle build."""
+    if kind not in ("orc", "fast", "bound"):
+        return
+    lib, _ = _cache[kind]
+    fn = lib.orc_bound_set_gp19_table if kind == "bound" else lib.orc_set_gp19_table
+    fn.restype = None
+    if _gp19 is None:
+        fn(0, None, None, 0, None, 0, None, 0, None, None, None)
+        return
+    t = _gp19
+    ax = [np.ascontiguousarray(t[k], dtype=np.float64) for k in _GP19_KEYS]
+    data = np.ascontiguousarray(t["data"], dtype=np.float64)
+    fn.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp]
+    fn(len(ax[0]), ax[0], ax[1], len(ax[2]), ax[2], len(ax[3]), ax[3], len(ax[4]), ax[4], ax[5],
+       data)
+    fn.argtypes = None
+
+
+def set_gp19_table(table):
+    """Installs (a dict with the axes nb [fm^-3], lognb, ye, T [MeV], w
+    [MeV], logw and data [nnb, nye, nT, nw]) or clears (None) the GP19 table
+    in every oracle build; without one, BREM_GP19 adds zero, as the
+    reference's table-free configuration does."""
+    global _gp19
+    _gp19 = None if table is None else dict(table)
+    for kind in list(_cache):
+        _apply_gp19(kind)
+
+
+def gp19_interp(nb, ye, T, w):
+    """gp19_interpolator of the installed table, elementwise (NaN outside)."""
+    a = [_f64(np.broadcast_to(v, np.broadcast(nb, ye, T, w).shape)) for v in (nb, ye, T, w)]
+    out = np.empty(a[0].size)
+    _fn("orc", "gp19_interp")(out.size, *[x.ravel() for x in a], out)
+    return out.reshape(a[0].shape)
+
+
+def brem_gp19(om, omp, nb, T, yp):
+    """BremKernelAbsGP19 (em, abs) of the installed table, elementwise."""
+    a = [_f64(np.broadcast_to(v, np.broadcast(om, omp, nb, T, yp).shape)).ravel()
+         for v in (om, omp, nb, T, yp)]
+    em, ab = np.empty(a[0].size), np.empty(a[0].size)
+    _fn("orc", "brem_gp19")(em.size, *a, em, ab)
+    return em, ab
 
 
 def _fn(kind, name):

@@ -98,6 +98,13 @@ using namespace orc;
 
 extern "C" {
 
+// the GP19 table of this library (orc_set_gp19_table of the double build)
+void orc_bound_set_gp19_table(int nnb, const double* nb, const double* lognb, int nye,
+                              const double* ye, int nT, const double* T, int nw, const double* w,
+                              const double* logw, const double* data) {
+  orc::gp19_assign(nnb, nb, lognb, nye, ye, nT, T, nw, w, logw, data);
+}
+
 // Same arguments as orc_m1_opacities (distr NULL: reconstruct from M1);
 // writes the outputs to out and their magnitudes m to mag ([20][np] each).
 int orc_bound_m1_opacities(int64_t np, int n1, const double* x1, const double* w1, int n2,

@@ -81,6 +81,31 @@ void orc_fermi_distr(int64_t n, const double* e, const double* T, const double* 
   for (int64_t i = 0; i < n; ++i) out[i] = fermi_distr(e[i], T[i], mu[i]);
 }
 
+void orc_set_gp19_table(int nnb, const double* nb, const double* lognb, int nye, const double* ye,
+                        int nT, const double* T, int nw, const double* w, const double* logw,
+                        const double* data) {
+  orc::gp19_assign(nnb, nb, lognb, nye, ye, nT, T, nw, w, logw, data);
+}
+
+// gp19_interpolator (nb [fm^-3], Ye, T, w [MeV]), elementwise
+void orc_gp19_interp(int64_t n, const double* nb, const double* ye, const double* T,
+                     const double* w, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = orc::gp19_interp<double>(nb[i], ye[i], T[i], w[i]);
+}
+
+// BremKernelAbsGP19 (omega, omega', nb [nm^-3], T, Ye = yp), elementwise
+void orc_brem_gp19(int64_t n, const double* om, const double* omp, const double* nb,
+                   const double* T, const double* yp, double* em, double* ab) {
+  for (int64_t i = 0; i < n; ++i) {
+    orc::Eos<double> e{};
+    e.nb = nb[i];
+    e.T = T[i];
+    e.yp = yp[i];
+    e.yn = 1. - yp[i];
+    orc::brem_gp19<double>(om[i], omp[i], e, em[i], ab[i]);
+  }
+}
+
 void orc_bessk1(int64_t n, const double* x, double* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = bessk1(x[i]);
 }

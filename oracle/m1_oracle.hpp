@@ -46,6 +46,7 @@
 #define BNS_NURATES_AMD_ORACLE_M1_ORACLE_HPP_
 
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -784,6 +785,172 @@ inline void brem_hr98(R omega, R omega_p, const Eos<R>& eos, bool medium, R& s_e
 }
 
 // ---------------------------------------------------------------------------
+// GP19 bremsstrahlung (kernel_brem_GP19.hpp:35-480) over a table the caller
+// supplies (orc_set_gp19_table): the reference's GP19Table.hpp blob is absent
+// from its tree.  Without a table the kernel adds zero, as the reference's
+// table-free configuration does (:483-493).  The electron fraction is the
+// point block's yp (the block carries no separate ye; GP19 is the only reader
+// of ye on the path).
+// ---------------------------------------------------------------------------
+struct Gp19Tab {
+  int nnb = 0, nye = 0, nT = 0, nw = 0;
+  std::vector<double> nb, lognb, ye, T, w, logw, data;  // data [nnb][nye][nT][nw]
+};
+inline Gp19Tab& gp19_table() {
+  static Gp19Tab t;
+  return t;
+}
+inline bool gp19_set() { return !gp19_table().data.empty(); }
+// installs (nnb > 0) or clears (nnb == 0) the table
+inline void gp19_assign(int nnb, const double* nb, const double* lognb, int nye, const double* ye,
+                        int nT, const double* T, int nw, const double* w, const double* logw,
+                        const double* data) {
+  Gp19Tab& t = gp19_table();
+  t = Gp19Tab();
+  if (nnb <= 0) return;
+  t.nnb = nnb;
+  t.nye = nye;
+  t.nT = nT;
+  t.nw = nw;
+  t.nb.assign(nb, nb + nnb);
+  t.lognb.assign(lognb, lognb + nnb);
+  t.ye.assign(ye, ye + nye);
+  t.T.assign(T, T + nT);
+  t.w.assign(w, w + nw);
+  t.logw.assign(logw, logw + nw);
+  t.data.assign(data, data + size_t(nnb) * nye * nT * nw);
+}
+
+// gp19_find_bracketing_indices, kernel_brem_GP19.hpp:34-56: the first axis
+// interval holding value (a linear scan); -1 outside the axis
+template <class R>
+inline int gp19_bracket(R value, const std::vector<double>& axis, int* i0, R* t) {
+  const int size = int(axis.size());
+  if (rval(value) < axis[0] || rval(value) > axis[size - 1]) return -1;
+  for (int i = 0; i < size - 1; ++i) {
+    if (rval(value) >= axis[i] && rval(value) <= axis[i + 1]) {
+      *i0 = i;
+      *t = (value - R(axis[i])) / (R(axis[i + 1]) - R(axis[i]));
+      return 0;
+    }
+  }
+  return -1;
+}
+
+// gp19_interpolator, kernel_brem_GP19.hpp:72-138: log-linear in nb and w,
+// linear in Ye and T; NaN outside the table
+template <class R>
+inline R gp19_interp(R nb, R Ye, R T, R w) {
+  const Gp19Tab& tb = gp19_table();
+  R log_nb = rlog(nb), log_w = rlog(w);
+  int i0 = 0, j0 = 0, k0 = 0, l0 = 0;
+  R tx = R(0.), ty = R(0.), tz = R(0.), tw = R(0.);
+  if (gp19_bracket(log_nb, tb.lognb, &i0, &tx) < 0 || gp19_bracket(Ye, tb.ye, &j0, &ty) < 0 ||
+      gp19_bracket(T, tb.T, &k0, &tz) < 0 || gp19_bracket(log_w, tb.logw, &l0, &tw) < 0)
+    return R(std::numeric_limits<double>::quiet_NaN());
+  auto D = [&](int i, int j, int k, int l) {
+    return R(tb.data[((size_t(i) * tb.nye + j) * tb.nT + k) * tb.nw + l]);
+  };
+  const int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1, l1 = l0 + 1;
+  R a0 = R(1.) - tx, a1 = tx, b0 = R(1.) - ty, b1 = ty;
+  R c0 = R(1.) - tz, c1 = tz, d0 = R(1.) - tw, d1 = tw;
+  return D(i0, j0, k0, l0) * a0 * b0 * c0 * d0 + D(i0, j0, k0, l1) * a0 * b0 * c0 * d1 +
+         D(i0, j0, k1, l0) * a0 * b0 * c1 * d0 + D(i0, j0, k1, l1) * a0 * b0 * c1 * d1 +
+         D(i0, j1, k0, l0) * a0 * b1 * c0 * d0 + D(i0, j1, k0, l1) * a0 * b1 * c0 * d1 +
+         D(i0, j1, k1, l0) * a0 * b1 * c1 * d0 + D(i0, j1, k1, l1) * a0 * b1 * c1 * d1 +
+         D(i1, j0, k0, l0) * a1 * b0 * c0 * d0 + D(i1, j0, k0, l1) * a1 * b0 * c0 * d1 +
+         D(i1, j0, k1, l0) * a1 * b0 * c1 * d0 + D(i1, j0, k1, l1) * a1 * b0 * c1 * d1 +
+         D(i1, j1, k0, l0) * a1 * b1 * c0 * d0 + D(i1, j1, k0, l1) * a1 * b1 * c0 * d1 +
+         D(i1, j1, k1, l0) * a1 * b1 * c1 * d0 + D(i1, j1, k1, l1) * a1 * b1 * c1 * d1;
+}
+
+// gp19_BremKernel_in_ranges, :158-170 (nb in fm^-3)
+template <class R>
+inline R gp19_in_ranges(R nb, R Ye, R T, R w) {
+  return R(k::BremConst) * (nb * R(1e18)) * gp19_interp(nb, Ye, T, w);
+}
+// gp19_LowDensityExtrapolation, :176-189
+template <class R>
+inline R gp19_low_density(R nb_below, R Ye, R T, R w) {
+  const R nb_min = R(gp19_table().nb[0]);
+  R r = nb_below / nb_min;
+  return gp19_in_ranges(nb_min, Ye, T, w) * (r * r);
+}
+// gp19_LowTemperatureExtrapolation, :192-212
+template <class R>
+inline R gp19_low_temperature(R nb, R Ye, R T_below, R w) {
+  R kT2 = gp19_in_ranges(nb, Ye, R(2.), w), kT4 = gp19_in_ranges(nb, Ye, R(4.), w);
+  R m = rlog(kT4 / kT2) * R(1.4426950408889634);
+  return kT2 * rpow(T_below * R(0.5), m);
+}
+// gp19_LowDensity_and_LowTemperatureExtrapolation, :216-258
+template <class R>
+inline R gp19_low_density_low_temperature(R nb_below, R Ye, R T_below, R w) {
+  const double eps = 1e-30;
+  R kT2 = gp19_low_density(nb_below, Ye, R(2.), w), kT4 = gp19_low_density(nb_below, Ye, R(4.), w);
+  if (rval(kT2) < eps) kT2 = R(eps);
+  if (rval(kT4) < eps) kT4 = R(eps);
+  R m = rlog(kT4 / kT2) * R(1.4426950408889634);
+  R k = kT2 * rpow(T_below * R(0.5), m);
+  if (rval(k) < eps) k = R(eps);
+  return k;
+}
+// gp19_HighTemperatureExtrapolation, :262-312
+template <class R>
+inline R gp19_high_temperature(R nb, R Ye, R T_above, R w) {
+  const R nb_nm = nb * R(1e18), nb_nm2 = nb_nm * nb_nm, w2 = w * w;
+  const R R48 = gp19_in_ranges(nb, Ye, R(48.), w), R50 = gp19_in_ranges(nb, Ye, R(50.), w);
+  const R K = R(4.) * R(k::BremConst) * nb_nm2;
+  const R A = R(1.) / R50, A2 = A * A;
+  const R B = R(1.) / (K * R(7.0710678118654755)), B2 = B * B;
+  const R m = (R50 - R48) / (R(50.) - R(48.));
+  const R beta = R(8.) * w2 * K * B2 / (R(2.) * m * A2 * R(7.0710678118654755) + A * K * B);
+  const R gamma = (A * beta - R(4.) * B * w2) / (B * nb_nm2 * R(50.));
+  const R lor1 = (w / T_above) * (w / T_above);
+  const R lor2 = (nb_nm2 / (R(4.) * T_above)) * gamma;
+  const R inv = R(1.) / (lor1 + lor2);
+  const R scal = (beta * nb_nm) / (T_above * rsqrt(T_above));
+  return R(k::BremConst) * nb_nm * (scal * inv);
+}
+// gp19_HighTemperature_and_LowDensityExtrapolation, :316-333
+template <class R>
+inline R gp19_high_temperature_low_density(R nb_below, R Ye, R T_above, R w) {
+  const R nb_min = R(gp19_table().nb[0]);
+  R r = nb_below / nb_min;
+  return gp19_high_temperature(nb_min, Ye, T_above, w) * (r * r);
+}
+// BremKernelAbsGP19, :354-480: clipping, the in-range interpolation or the
+// extrapolations, the l = 0 factor 3, |.|, detailed balance
+template <class R>
+inline void brem_gp19(R omega, R omega_p, const Eos<R>& eos, R& s_em, R& s_abs) {
+  const Gp19Tab& tb = gp19_table();
+  R T = eos.T, Ye = eos.yp, w = omega + omega_p;
+  const R w_original = w;
+  R nb = eos.nb * R(1e-18);
+  const double nb_max = tb.nb[tb.nnb - 1], nb_min = tb.nb[0], Ye_max = tb.ye[tb.nye - 1];
+  const double T_max = tb.T[tb.nT - 1], T_min = tb.T[0];
+  const double w_max = tb.w[tb.nw - 1], w_min = tb.w[0];
+  if (rval(Ye) > Ye_max) Ye = R(Ye_max);
+  if (rval(nb) > nb_max) nb = R(nb_max);
+  if (rval(w) > w_max) w = R(w_max);
+  else if (rval(w) < w_min) w = R(w_min);
+  R sa;
+  if (rval(nb) >= nb_min && rval(T) >= T_min && rval(T) <= T_max) sa = gp19_in_ranges(nb, Ye, T, w);
+  else if (rval(nb) < nb_min) {
+    if (rval(T) < T_min) sa = gp19_low_density_low_temperature(nb, Ye, T, w);
+    else if (rval(T) > T_max) sa = gp19_high_temperature_low_density(nb, Ye, T, w);
+    else sa = gp19_low_density(nb, Ye, T, w);
+  } else if (rval(T) < T_min) {
+    sa = gp19_low_temperature(nb, Ye, T, w);
+  } else {
+    sa = gp19_high_temperature(nb, Ye, T, w);
+  }
+  sa = R(3.) * sa;
+  s_abs = rfabs(sa);
+  s_em = s_abs * safe_exp(-w_original / T);
+}
+
+// ---------------------------------------------------------------------------
 // Inelastic scattering on e- / e+ (kernel_nes.hpp).
 // G[k][a] are F_k at a = 0: eta-wp, 1: eta-w, 2: eta-|w-wp|, 3: eta.
 // ---------------------------------------------------------------------------
@@ -1194,11 +1361,12 @@ inline void double_integrand(const Quad& q, R t, const Eos<R>& eos, const Distr<
       }
     }
   }
-  if (fl.use_brem == 1 && pr.brem_impl != BREM_GP19) {
+  if (fl.use_brem == 1 && (pr.brem_impl != BREM_GP19 || gp19_set())) {
     for (int a = 0; a < N; ++a) {
       for (int b = a; b < N; ++b) {
         R se, sa;
         if (pr.brem_impl == BREM_BRT06) brem_brt06(nu[a], nu[b], eos, se, sa);
+        else if (pr.brem_impl == BREM_GP19) brem_gp19(nu[a], nu[b], eos, se, sa);
         else brem_hr98(nu[a], nu[b], eos, pr.use_NN_medium_corr != 0, se, sa);
         for (int s = 0; s < 4; ++s) {
           M.E(s, a, b) += se;
@@ -1563,8 +1731,9 @@ inline void spectral_integrand_pb(R nu, int a, const SpecPoint<R>& S, const R Fy
     R se = R(0.), sa = R(0.);
     if (pr.brem_impl == BREM_BRT06) brem_brt06(nu, nub, eos, se, sa);
     else if (pr.brem_impl == BREM_HR98) brem_hr98(nu, nub, eos, pr.use_NN_medium_corr != 0, se, sa);
-    // BREM_GP19: BremKernelAbsGP19 is zero in the reference's table-less
-    // configuration (kernel_brem_GP19.hpp:483-493)
+    else if (gp19_set()) brem_gp19(nu, nub, eos, se, sa);
+    // BREM_GP19 without a table: BremKernelAbsGP19 is zero in the
+    // reference's table-free configuration (kernel_brem_GP19.hpp:483-493)
     for (int s = 0; s < 4; ++s) {
       bk.em[s] = se;
       bk.abs[s] = sa;

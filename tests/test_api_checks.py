@@ -81,3 +81,14 @@ def test_host_output_array_is_checked():
     with pytest.raises(api.BnsAmdError, match="out"):
         c.compute_m1_opacities(pts, out=np.zeros((20, 8), np.float32))
     assert "bnsamd_m1_opacities_host" not in c.lib.calls
+
+
+def test_gp19_table_shape_is_checked():
+    import numpy as np
+    from tests import gp19_synth as G
+    c = _ctx()
+    t = G.table("smooth")
+    t["data"] = t["data"][:, :, :, :-1]
+    with pytest.raises(api.BnsAmdError, match="gp19"):
+        c.set_gp19_table(t)
+    assert "bnsamd_set_gp19_table" not in c.lib.calls
