@@ -132,6 +132,9 @@ k_spectral_f64(SpectralKernelArgs A) {
   __shared__ double s_Fe[2][6], s_fd0[2];
   __shared__ HrChan s_hc[3];
   __shared__ double s_brem[6];  // BRT06 prefactor; HR98 y, e^-y/12, p, -1/p, medium
+  // per-point constants of the beta and isoenergetic terms: EtaNNAbs n -> p,
+  // p -> n, dQ + dU, mu_p + mu_e - mu_n; EtaNNSc of protons and neutrons
+  __shared__ double s_bpt[4], s_etasc[2];
 
   const int lane = threadIdx.x;
   const int n = NQ > 0 ? NQ : A.n, N = 2 * n, E = ECH > 0 ? ECH : A.ech;
@@ -209,6 +212,20 @@ k_spectral_f64(SpectralKernelArgs A) {
   }
   // the bremsstrahlung point constants stay in LDS and are read in the cell
   // rounds where used (held in registers through the chunk loop they spill)
+  // beta_point / eta_nn_sc depend on the point only: once here, not per chunk
+  if (lane == 1) {
+    const PointParams& pp = A.params;
+    const BetaPoint bp = beta_point(vin[0], T, vin[2], vin[3], vin[4], vin[5], mu_e, vin[7],
+                                    vin[8], pp.use_dU != 0, pp.use_dm_eff != 0, pp.use_WM_ab != 0,
+                                    pp.use_decay != 0);
+    s_bpt[0] = bp.etanp;
+    s_bpt[1] = bp.etapn;
+    s_bpt[2] = bp.Qp;
+    s_bpt[3] = bp.munu;
+  } else if (lane == 2) {
+    s_etasc[0] = eta_nn_sc(vin[0], T, vin[2]);
+    s_etasc[1] = eta_nn_sc(vin[0], T, vin[3]);
+  }
   if (lane == 0) {
     s_brem[0] = brt_pre;
     s_brem[1] = hp.y;
@@ -293,50 +310,58 @@ k_spectral_f64(SpectralKernelArgs A) {
       for (int k = 1; k <= 5; ++k) V[dst + (k - 1) * E] = F[k];
     }
     // ---- distribution values, beta and iso at the energies ----------------------
+    // one task per (energy, species) for the distribution values and the
+    // beta terms, one per energy for the isoenergetic term: the three kinds
+    // diverge within a round, so each kind's cost is one task's
     {
       const PointParams& pp = A.params;
-      for (int t = lane; t < 3 * ne; t += kWave) {
-        {
-          const int e = t / 3, kind = t - 3 * e;
-          const double nu = energy(e);
-          if (kind == 0) {
-            // TotalNuF(nu) of the four species (m1_opacities.hpp:1533-1536)
-            V[L.nu + e] = nu;
-            V[L.y + e] = over_T(nu);
-            const double lw = log(nu);
-            V[L.lnu + e] = lw;
-  #pragma unroll
-            for (int s = 0; s < 4; ++s) V[L.gnu + s * E + e] = nuf(nu, lw, Dget(s));
-          } else if (kind == 1) {
-            // AbsOpacity (formulas_beta_iso.hpp:291-319), nue and anue
-            double em[2] = {0., 0.}, ab[2] = {0., 0.};
-            if (A.flags.use_abs_em != 0) {
-              const BetaPoint bp =
-                  beta_point(vin[0], T, vin[2], vin[3], vin[4], vin[5], mu_e, vin[7], vin[8],
-                             pp.use_dU != 0, pp.use_dm_eff != 0, pp.use_WM_ab != 0,
-                             pp.use_decay != 0);
-              beta_em_ab(nu, 0, bp, em[0], ab[0]);
-              beta_em_ab(nu, 1, bp, em[1], ab[1]);
-            }
-  #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-              V[L.bem + s * E + e] = em[s];
-              V[L.bab + s * E + e] = ab[s];
-            }
-          } else {
-            // IsoScattLegCoeff, l = 0 (formulas_beta_iso.hpp:524-560)
-            double iso = 0.;
-            if (A.flags.use_iso != 0) {
-              const double etap = eta_nn_sc(vin[0], T, vin[2]), etan = eta_nn_sc(vin[0], T, vin[3]);
-              double r0p = 1., r0n = 1.;
-              if (pp.use_WM_sc) {
-                r0p = wm_scatt_r0(nu, 1);
-                r0n = wm_scatt_r0(nu, 2);
-              }
-              iso = etap * c::IsoC0p * r0p + etan * c::IsoC0n * r0n;
-            }
-            V[L.iso + e] = iso;
+      for (int t = lane; t < 4 * ne; t += kWave) {
+        // TotalNuF(nu) of species s (m1_opacities.hpp:1533-1536)
+        const int e = t >> 2, sp = t & 3;
+        const double nu = energy(e);
+        const double lw = log(nu);
+        if (sp == 0) {
+          V[L.nu + e] = nu;
+          V[L.y + e] = over_T(nu);
+          V[L.lnu + e] = lw;
+        }
+        V[L.gnu + sp * E + e] = nuf(nu, lw, Dget(sp));
+      }
+      const int nbeta = 2 * ne;
+      for (int t = lane; t < nbeta + ne; t += kWave) {
+        if (t < nbeta) {
+          // AbsOpacity (formulas_beta_iso.hpp:291-319) of nue (sp 0) or anue (1)
+          const int e = t >> 1, sp = t & 1;
+          double em = 0., ab = 0.;
+          if (A.flags.use_abs_em != 0) {
+            BetaPoint bp;
+            bp.etanp = s_bpt[0];
+            bp.etapn = s_bpt[1];
+            bp.Qp = s_bpt[2];
+            bp.munu = s_bpt[3];
+            bp.mu_e = mu_e;
+            bp.T = T;
+            bp.rT = vin[kInvT];
+            bp.wm = pp.use_WM_ab != 0;
+            bp.decay = pp.use_decay != 0;
+            beta_em_ab(energy(e), sp, bp, em, ab);
           }
+          V[L.bem + sp * E + e] = em;
+          V[L.bab + sp * E + e] = ab;
+        } else {
+          // IsoScattLegCoeff, l = 0 (formulas_beta_iso.hpp:524-560)
+          const int e = t - nbeta;
+          double iso = 0.;
+          if (A.flags.use_iso != 0) {
+            const double nu = energy(e);
+            double r0p = 1., r0n = 1.;
+            if (pp.use_WM_sc) {
+              r0p = wm_scatt_r0(nu, 1);
+              r0n = wm_scatt_r0(nu, 2);
+            }
+            iso = s_etasc[0] * c::IsoC0p * r0p + s_etasc[1] * c::IsoC0n * r0n;
+          }
+          V[L.iso + e] = iso;
         }
       }
     }
