@@ -429,10 +429,13 @@ def config0_eval(ctx, steps=20, repeats=5):
     return res
 
 
-def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0):
+def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0, layouts=("struct", "fields")):
     """The spectral leg: events-timed passes of the spectral kernel over the
-    spectral workload (inputs resident in HBM), an oracle parity sample, and
-    the CPU restatement timed on a bounded prefix of the points."""
+    spectral workload (inputs resident in HBM) in each output layout -- one
+    SpectralOpacities record per (point, energy) (bnsamd_spectral_opacities_aos)
+    and the field-major [16][nnu][npts] block (bnsamd_spectral_opacities) --,
+    an oracle parity sample of each, and the CPU restatement timed on a
+    bounded prefix of the points.  The first layout is the line's figure."""
     import torch
     import oracle as O
     from bns_nurates_amd import workloads as W
@@ -442,48 +445,65 @@ def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0):
     dnu = torch.from_numpy(SPECTRAL_NU).cuda()
     prm = W.params_default_all(W.BREM_BRT06)
     ctx.set_quadrature(SPECTRAL_NQUAD)
-    out = torch.empty((16 * nnu, npts), dtype=torch.float64, device="cuda")
-
-    def run():
-        ctx.compute_spectral_opacities(dsp, dnu, W.FLAGS_ALL, prm, stim_abs=0, out=out)
-
-    for _ in range(warmup):
-        run()
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
-    log(f"spectral: timing {steps} steps ({npts} points x {nnu} energies)")
-    t0 = time.perf_counter()
-    for a, b in ev:
-        a.record()
-        run()
-        b.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    nev = npts * nnu
-    achieved = SPECTRAL_FLOPS_PER_EVAL * nev / (kern_ms * 1e-3) / 1e12
-    o = out.view(16, nnu, npts).cpu().numpy()
-    assert np.isfinite(o).all()
     idx = np.arange(0, npts, npts // 32)[:32]
     ref, mag = O.bound_spectral_opacities(np.ascontiguousarray(sp[:, idx]), SPECTRAL_NU,
                                           SPECTRAL_NQUAD, W.FLAGS_ALL, prm, stim=0)
-    excess = float(O.parity_excess(o[:, :, idx], ref, mag).max())
+    nev = npts * nnu
+    legs = {}
+    for layout in layouts:
+        if layout == "struct":
+            out = torch.empty((npts, nnu, 16), dtype=torch.float64, device="cuda")
+        else:
+            out = torch.empty((16 * nnu, npts), dtype=torch.float64, device="cuda")
+
+        def run():
+            ctx.compute_spectral_opacities(dsp, dnu, W.FLAGS_ALL, prm, stim_abs=0, out=out,
+                                           layout=layout)
+
+        for _ in range(warmup):
+            run()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        log(f"spectral ({layout} layout): timing {steps} steps ({npts} points x {nnu} energies)")
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record()
+            run()
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        o = out.cpu().numpy()
+        o = o.transpose(2, 1, 0) if layout == "struct" else o.reshape(16, nnu, npts)
+        assert np.isfinite(o).all()
+        excess = float(O.parity_excess(o[:, :, idx], ref, mag).max())
+        achieved = SPECTRAL_FLOPS_PER_EVAL * nev / (kern_ms * 1e-3) / 1e12
+        legs[layout] = {"value": nev * steps / wall, "kernel_ms": kern_ms, "achieved": achieved,
+                        "bar_excess_max": excess}
+        log(f"spectral ({layout}): {legs[layout]['value']:.4g} evals/s, kernel {kern_ms:.2f} ms, "
+            f"{achieved:.2f} TFLOP/s, parity excess {excess:.3g}")
+        del out
+    main = legs[layouts[0]]
     res = {"workload": f"spectral_grid32_e100_n10_brt06: {npts} points of the grid workload x "
                        f"{nnu} energies 10^(-1+4k/100) MeV (the reference spectral test's grid), "
                        "n=10, all reactions, BRT06, default_all, not stimulated, distributions "
                        "reconstructed from M1",
-           "value": nev * steps / wall, "unit": "(point, energy) evaluations/s",
-           "steps": steps, "kernel_ms": kern_ms,
-           "roofline": {"bound": "fp64 vector ALU", "achieved": achieved,
+           "layout": {"struct": "SpectralOpacities[npts][nnu] (bnsamd_spectral_opacities_aos)",
+                      "fields": "[16][nnu][npts] (bnsamd_spectral_opacities)"}[layouts[0]],
+           "value": main["value"], "unit": "(point, energy) evaluations/s",
+           "steps": steps, "kernel_ms": main["kernel_ms"],
+           "roofline": {"bound": "fp64 vector ALU", "achieved": main["achieved"],
                         "peak": PEAK_FP64_VECTOR_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK_FP64_VECTOR_TFLOPS, "kernel": "k_spectral_f64",
+                        "frac": main["achieved"] / PEAK_FP64_VECTOR_TFLOPS,
+                        "kernel": "k_spectral_f64",
                         "flops_per_eval": SPECTRAL_FLOPS_PER_EVAL,
                         "transcendentals_per_eval": SPECTRAL_TRANSCENDENTALS_PER_EVAL,
                         "algorithmic_bytes_per_eval": 16 * 8 + 8 * 21 / nnu},
-           "parity_sample": {"points": len(idx), "energies": nnu, "bar_excess_max": excess}}
-    log(f"spectral: {res['value']:.4g} evals/s, kernel {kern_ms:.2f} ms, "
-        f"{achieved:.2f} TFLOP/s, parity excess {excess:.3g}")
+           "parity_sample": {"points": len(idx), "energies": nnu,
+                             "bar_excess_max": main["bar_excess_max"]},
+           "layouts": {k: {"kernel_ms": v["kernel_ms"], "value": v["value"],
+                           "bar_excess_max": v["bar_excess_max"]} for k, v in legs.items()}}
     if cpu:
         kind = "fast" if O.available("fast") else "orc"
         threads = cpu_threads()
@@ -649,6 +669,9 @@ def main():
     ap.add_argument("--surrogate-epochs", type=int, default=48)
     ap.add_argument("--surrogate-ln-loss", action="store_true",
                     help="train the learned form on the squared error in natural-log units")
+    ap.add_argument("--spectral-layout", choices=("both", "struct", "fields"), default="both",
+                    help="output layouts of the spectral leg (struct: SpectralOpacities records; "
+                         "fields: [16][nnu][npts]); the first timed is the line's figure")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -796,8 +819,10 @@ def main():
     if not args.no_cpu:
         line["config0"] = config0_eval(ctx)
     if not args.no_spectral:
+        lay = {"both": ("struct", "fields"), "struct": ("struct",),
+               "fields": ("fields",)}[args.spectral_layout]
         line["spectral"] = spectral_eval(ctx, max(3, args.steps // 2), 1,
-                                         cpu=not args.no_cpu)
+                                         cpu=not args.no_cpu, layouts=lay)
     if not args.no_serving:
         line["serving"] = serving_sweep()
     if not args.no_surrogate:

@@ -177,6 +177,9 @@ def load_library():
                                              C.POINTER(_Params), C.c_int, C.c_int, vp], C.c_int),
         "bnsamd_spectral_opacities": ([vp, i64, i64, vp, vp, i64, vp, i64, C.POINTER(_Flags),
                                        C.POINTER(_Params), C.c_int, vp], C.c_int),
+        "bnsamd_spectral_opacities_aos": ([vp, i64, i64, vp, vp, i64, vp, i64,
+                                           C.POINTER(_Flags), C.POINTER(_Params), C.c_int, vp],
+                                          C.c_int),
         "bnsamd_spectral_opacities_host": ([vp, i64, vp, vp, i64, vp, C.c_int,
                                             C.POINTER(_Flags), C.POINTER(_Params), C.c_int,
                                             vp], C.c_int),
@@ -387,7 +390,7 @@ class Context:
         return out
 
     def compute_spectral_opacities(self, pts, nu, flags=None, params=None, stim_abs: int = 0,
-                                   distr=None, out=None):
+                                   distr=None, out=None, layout: str = "fields"):
         """Spectral opacities [16, nnu, npts] (j, kappa, j_s, kappa_s of the
         four species; field order of SpectralOpacities) of the point block
         ``pts`` [21, npts] at the energies ``nu``: [nnu] shared by all points
@@ -398,9 +401,15 @@ class Context:
         Device tensors run asynchronously on the context stream (``out`` a
         [16 * nnu, ld] tensor, allocated when None, returned viewed as
         [16, nnu, ld]); numpy arrays go through the synchronous host entry
-        point."""
+        point.  ``layout="struct"`` (device tensors): one SpectralOpacities
+        record per (point, energy), ``out`` a contiguous [npts, nnu, 16]
+        tensor (bnsamd_spectral_opacities_aos)."""
+        if layout not in ("fields", "struct"):
+            raise BnsAmdError("layout must be 'fields' or 'struct'")
         fl, pr = as_flags(flags).c(), as_params(params).c()
         if isinstance(pts, np.ndarray):
+            if layout != "fields":
+                raise BnsAmdError("layout='struct' takes device tensors")
             pts = np.ascontiguousarray(pts, dtype=np.float64)
             n = pts.shape[1]
             nu = np.ascontiguousarray(nu, dtype=np.float64)
@@ -438,6 +447,17 @@ class Context:
             dp, ldd = _tensor_ptr(distr, DI_NFIELDS, n, "distr")
             if ldd != ld:
                 raise BnsAmdError("distr must have the leading dimension of pts")
+        if layout == "struct":
+            if out is None:
+                out = torch.empty((n, nnu, SP_NFIELDS), dtype=torch.float64, device=pts.device)
+            if (out.dtype != torch.float64 or not out.is_cuda or not out.is_contiguous()
+                    or tuple(out.shape) != (n, nnu, SP_NFIELDS)):
+                raise BnsAmdError(f"out: contiguous float64 device tensor [{n}, {nnu}, "
+                                  f"{SP_NFIELDS}] required")
+            self._check(self.lib.bnsamd_spectral_opacities_aos(
+                self.h, n, ld, pp, dp, nnu, nu.data_ptr(), nu_ld, C.byref(fl), C.byref(pr),
+                int(stim_abs), out.data_ptr()), "bnsamd_spectral_opacities_aos")
+            return out
         if out is None:
             out = torch.empty((SP_NFIELDS * nnu, ld), dtype=torch.float64, device=pts.device)
         op, ldo = _tensor_ptr(out, SP_NFIELDS * nnu, n, "out")

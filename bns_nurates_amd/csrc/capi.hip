@@ -261,10 +261,12 @@ int run_gray(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pts,
 int run_spectral(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pts,
                  const double* d_distr, int64_t nnu, const double* d_nu, int64_t nu_ld,
                  const bnsamd_flags* flags, const bnsamd_params* params, int stim_abs,
-                 double* d_out) {
+                 double* d_out, bool aos = false) {
   if (npts < 0 || ld < npts || nnu < 0 || !d_pts || !d_nu || !d_out || !flags || !params ||
       (nu_ld != 0 && nu_ld < npts))
     return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities: invalid argument");
+  if (aos && npts > INT64_MAX / BNSAMD_SP_NFIELDS / (nnu > 0 ? nnu : 1))
+    return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities_aos: output too large");
   if (stim_abs != 0 && stim_abs != 1)
     return fail(c, BNSAMD_ERR_INVALID, "bnsamd_spectral_opacities: stim_abs must be 0 or 1");
   if (params->brem_implementation < 0 || params->brem_implementation > 2)
@@ -281,6 +283,7 @@ int run_spectral(bnsamd_context* c, int64_t npts, int64_t ld, const double* d_pt
   a.nu = d_nu;
   a.nu_ld = nu_ld;
   a.out = d_out;
+  a.aos = aos ? 1 : 0;
   a.q1 = c->current->dev + L.q1;
   a.n = L.n1;
   a.ech = bns::spectral_energies_per_chunk(L.n1);
@@ -574,6 +577,17 @@ int bnsamd_spectral_opacities(bnsamd_context* c, int64_t npts, int64_t ld, const
   (void)hipSetDevice(c->device);
   return run_spectral(c, npts, ld, d_pts, d_distr, nnu, d_nu, nu_ld, flags, params, stim_abs,
                       d_out);
+}
+
+int bnsamd_spectral_opacities_aos(bnsamd_context* c, int64_t npts, int64_t ld,
+                                  const double* d_pts, const double* d_distr, int64_t nnu,
+                                  const double* d_nu, int64_t nu_ld, const bnsamd_flags* flags,
+                                  const bnsamd_params* params, int stim_abs, double* d_out) {
+  if (check_ctx(c)) return BNSAMD_ERR_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  return run_spectral(c, npts, ld, d_pts, d_distr, nnu, d_nu, nu_ld, flags, params, stim_abs,
+                      d_out, true);
 }
 
 int bnsamd_spectral_opacities_host(bnsamd_context* c, int64_t npts, const double* h_pts,

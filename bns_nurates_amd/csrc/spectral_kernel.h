@@ -19,7 +19,9 @@ struct SpectralKernelArgs {
   int64_t nnu;           // energies per point
   const double* nu;      // energies [MeV]: nu[e] (nu_ld == 0) or nu[e * nu_ld + i]
   int64_t nu_ld;
-  double* out;           // [16][nnu][ld]: out[(f * nnu + e) * ld + i]
+  double* out;           // aos == 0: [16][nnu][ld], out[(f * nnu + e) * ld + i];
+                         // aos == 1: SpectralOpacities[npts][nnu], out[(i * nnu + e) * 16 + f]
+  int aos;
   const double* q1;      // 1D quadrature records (x, w) on [0, 1] (device)
   int n;                 // 1D quadrature order, 1..32
   int ech;               // energies per chunk (the LDS energy block)

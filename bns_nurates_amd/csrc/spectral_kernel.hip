@@ -235,6 +235,10 @@ k_spectral_f64(SpectralKernelArgs A) {
     s_brem[5] = hp.medium;
   }
   const volatile double* vbrem = s_brem;
+  // read where used, each chunk: loads the compiler may not hoist out of the
+  // chunk loop (held in registers through it they spill)
+  const volatile double* vbpt = s_bpt;
+  const volatile double* vetasc = s_etasc;
   __syncthreads();
   const volatile Spec* vD = s_D;
   auto Dget = [&](int s) {
@@ -294,13 +298,18 @@ k_spectral_f64(SpectralKernelArgs A) {
 
   for (int64_t chunk = c_begin; chunk < c_end; ++chunk) {
     const int64_t e0 = chunk * E;
+    // the lane index as a value of this iteration: keeps the compiler from
+    // hoisting the per-lane task indices and addresses of the energy phase
+    // out of the chunk loop, where they would be held (and spilled) through it
+    int cl = lane;
+    __asm__ volatile("" : "+v"(cl));
     const int ne = static_cast<int>(A.nnu - e0 < E ? A.nnu - e0 : E);
     auto energy = [&](int e) {
       return A.nu_ld ? A.nu[(e0 + e) * A.nu_ld + p] : A.nu[e0 + e];
     };
 
     // ---- Fermi-integral bundles F_1..F_5 at eta - y, eta + y, -eta - y ---------
-    for (int t = lane; t < 3 * ne; t += kWave) {
+    for (int t = cl; t < 3 * ne; t += kWave) {
       const int e = t / 3, kind = t - 3 * e;
       const double y = over_T(energy(e));
       double F[6];
@@ -315,7 +324,7 @@ k_spectral_f64(SpectralKernelArgs A) {
     // diverge within a round, so each kind's cost is one task's
     {
       const PointParams& pp = A.params;
-      for (int t = lane; t < 4 * ne; t += kWave) {
+      for (int t = cl; t < 4 * ne; t += kWave) {
         // TotalNuF(nu) of species s (m1_opacities.hpp:1533-1536)
         const int e = t >> 2, sp = t & 3;
         const double nu = energy(e);
@@ -328,17 +337,17 @@ k_spectral_f64(SpectralKernelArgs A) {
         V[L.gnu + sp * E + e] = nuf(nu, lw, Dget(sp));
       }
       const int nbeta = 2 * ne;
-      for (int t = lane; t < nbeta + ne; t += kWave) {
+      for (int t = cl; t < nbeta + ne; t += kWave) {
         if (t < nbeta) {
           // AbsOpacity (formulas_beta_iso.hpp:291-319) of nue (sp 0) or anue (1)
           const int e = t >> 1, sp = t & 1;
           double em = 0., ab = 0.;
           if (A.flags.use_abs_em != 0) {
             BetaPoint bp;
-            bp.etanp = s_bpt[0];
-            bp.etapn = s_bpt[1];
-            bp.Qp = s_bpt[2];
-            bp.munu = s_bpt[3];
+            bp.etanp = vbpt[0];
+            bp.etapn = vbpt[1];
+            bp.Qp = vbpt[2];
+            bp.munu = vbpt[3];
             bp.mu_e = mu_e;
             bp.T = T;
             bp.rT = vin[kInvT];
@@ -359,7 +368,7 @@ k_spectral_f64(SpectralKernelArgs A) {
               r0p = wm_scatt_r0(nu, 1);
               r0n = wm_scatt_r0(nu, 2);
             }
-            iso = s_etasc[0] * c::IsoC0p * r0p + s_etasc[1] * c::IsoC0n * r0n;
+            iso = vetasc[0] * c::IsoC0p * r0p + vetasc[1] * c::IsoC0n * r0n;
           }
           V[L.iso + e] = iso;
         }
@@ -614,11 +623,15 @@ k_spectral_f64(SpectralKernelArgs A) {
 
     // ============================ assemble + store ============================
     // ComputeSpectralOpacitiesNotStimulatedAbs (m1_opacities.hpp:1572-1641);
-    // the stimulated form adds j / c to kappa (:1644-1660)
+    // the stimulated form adds j / c to kappa (:1644-1660).  Field-major
+    // output: lane t writes field t / ne of energy t % ne, one 8-byte value
+    // per row of the block; struct output: the chunk's 16 ne values are one
+    // contiguous run, lane t writes field t % 16 of energy t / 16.
     {
       const double F = c::FourPiHc3, cl = c::Clight, four_pi = 4. * c::Pi;
+      const bool aos = A.aos != 0;
       for (int t = lane; t < 16 * ne; t += kWave) {
-        const int f = t / ne, e = t - f * ne;
+        const int f = aos ? t & 15 : t / ne, e = aos ? t >> 4 : t - f * ne;
         const int s = f & 3, kind = f >> 2;
         const double nu = V[L.nu + e];
         const double emb = s < 2 ? V[L.bem + s * E + e] : 0.;
@@ -633,7 +646,8 @@ k_spectral_f64(SpectralKernelArgs A) {
           ks = js / cl + ks;
         }
         const double v = kind == 0 ? j : kind == 1 ? kap : kind == 2 ? js : ks;
-        A.out[(f * A.nnu + e0 + e) * ld + p] = v;
+        if (aos) A.out[(p * A.nnu + e0 + e) * 16 + f] = v;
+        else A.out[(f * A.nnu + e0 + e) * ld + p] = v;
       }
     }
     __syncthreads();  // the next chunk rewrites the energy block
@@ -675,7 +689,7 @@ hipError_t launch_spectral_f64(const SpectralKernelArgs& a, hipStream_t stream) 
   dim3 grid(static_cast<unsigned>(groups)), block(kWave);
   const size_t lds = spectral_lds_bytes(a.n, a.ech);
   if (a.gp19 && a.params.brem_impl == 2 && a.flags.use_brem != 0)  // GP19 over a table
-    hipLaunchKernelGGL((k_spectral_f64<2, 0, 0, true>), grid, block, lds, stream, a);
+    hipLaunchKernelGGL((k_spectral_f64<3, 0, 0, true>), grid, block, lds, stream, a);
   else if (a.n == kFixedN && a.ech == kFixedE)
     hipLaunchKernelGGL((k_spectral_f64<4, kFixedN, kFixedE>), grid, block, lds, stream, a);
   else

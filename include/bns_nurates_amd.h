@@ -185,7 +185,20 @@ int bnsamd_spectral_opacities(bnsamd_context* ctx, int64_t npts, int64_t ld, con
                               int64_t nu_ld, const bnsamd_flags* flags,
                               const bnsamd_params* params, int stim_abs, double* d_out);
 
-/* Same, from and to host memory (synchronous; ld = npts).  h_nu holds nnu
+/* Same, with the output as an array of SpectralOpacities structs
+ * (bns_nurates.hpp:517-523), one per (point, energy): d_out[npts][nnu][16],
+ * value of field f (BNSAMD_SP_*) of energy e at point i at
+ * d_out[(i * nnu + e) * 16 + f] -- what the reference's drivers hold when they
+ * keep the struct that ComputeSpectralOpacities*Abs returns per (energy,
+ * point) (tests/tests.hpp:1174-1176).  Each point's values are one contiguous
+ * run, so every cache line of the output is written whole by one wavefront. */
+int bnsamd_spectral_opacities_aos(bnsamd_context* ctx, int64_t npts, int64_t ld,
+                                  const double* d_pts, const double* d_distr, int64_t nnu,
+                                  const double* d_nu, int64_t nu_ld, const bnsamd_flags* flags,
+                                  const bnsamd_params* params, int stim_abs, double* d_out);
+
+/* Same as bnsamd_spectral_opacities, from and to host memory (synchronous;
+ * ld = npts).  h_nu holds nnu
  * energies shared by all points (per_point_energies = 0) or nnu * npts
  * (per_point_energies = 1, energy e of point i at h_nu[e * npts + i]). */
 int bnsamd_spectral_opacities_host(bnsamd_context* ctx, int64_t npts, const double* h_pts,

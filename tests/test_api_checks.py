@@ -92,3 +92,14 @@ def test_gp19_table_shape_is_checked():
     with pytest.raises(api.BnsAmdError, match="gp19"):
         c.set_gp19_table(t)
     assert "bnsamd_set_gp19_table" not in c.lib.calls
+
+
+def test_spectral_layout_is_checked():
+    import numpy as np
+    c = _ctx()
+    pts = np.zeros((api.PT_NFIELDS, 4))
+    with pytest.raises(api.BnsAmdError, match="layout"):
+        c.compute_spectral_opacities(pts, np.ones(3), layout="points")
+    with pytest.raises(api.BnsAmdError, match="device tensors"):
+        c.compute_spectral_opacities(pts, np.ones(3), layout="struct")
+    assert not c.lib.calls

@@ -166,3 +166,27 @@ def test_grid_scale_properties(ctx):
     sub = np.ascontiguousarray(pts[:, idx])
     ref, m = O.bound_spectral_opacities(sub, nu, 10, W.FLAGS_ALL, prm, stim=0)
     assert parity("grid sample", o0[:, :, idx], ref, m) <= 1.0
+
+
+@pytest.mark.parametrize("nquad", [10, 7])
+def test_struct_layout_matches_field_layout(ctx, nquad):
+    """bnsamd_spectral_opacities_aos writes the values of
+    bnsamd_spectral_opacities as SpectralOpacities records [npts][nnu][16],
+    bit for bit: a ragged last chunk (nnu = 37), shared and per-point
+    energies, both stimulated forms, the fixed n = 10 instance and another
+    order."""
+    pts = W.make_points(70, seed=5)
+    nu = np.exp(np.linspace(np.log(0.2), np.log(400.0), 37))
+    prm = W.params_default_all(W.BREM_BRT06)
+    ctx.set_quadrature(nquad)
+    dpts = dev(pts)
+    for stim in (0, 1):
+        for nuv in (nu, nu[:, None] * (1.0 + 0.003 * np.arange(70))[None, :]):
+            say(f"struct layout: n={nquad} stim={stim} nu{nuv.shape}")
+            dnu = dev(nuv)
+            f = ctx.compute_spectral_opacities(dpts, dnu, W.FLAGS_ALL, prm, stim_abs=stim)
+            s = ctx.compute_spectral_opacities(dpts, dnu, W.FLAGS_ALL, prm, stim_abs=stim,
+                                               layout="struct")
+            torch.cuda.synchronize()
+            assert tuple(s.shape) == (70, 37, 16)
+            assert np.array_equal(f.cpu().numpy(), s.cpu().numpy().transpose(2, 1, 0))
