@@ -64,7 +64,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   const int64_t ld = A.ld;
   const double* __restrict__ tb = A.tab;
   const TableLayout L = NQ > 0 ? gray_table_layout(NQ, NQ) : A.L;
-  const volatile double* vin = s_in;
+  lds_cv<double>* vin = lds_volatile(s_in);
 
   // ---- point inputs: lane f loads field f ----------------------------------
   if (lane < 21) s_in[lane] = A.pts[lane * ld + p];
@@ -103,7 +103,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     }
   }
   __syncthreads();
-  const volatile Spec* vD = s_D;
+  lds_cv<Spec>* vD = lds_volatile(s_D);
   auto Dget = [&](int s) {
     Spec r;
     r.wt = vD[s].wt;
@@ -116,7 +116,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     r.lbf = vD[s].lbf;
     return r;
   };
-  const volatile double (*vFe)[6] = s_Fe;
+  lds_cv<double> (*vFe)[6] = (lds_cv<double> (*)[6])s_Fe;
   // anux with the distribution of nux: its TotalNuF values are copied
   const bool same23 = __builtin_amdgcn_readfirstlane(spec_same(Dget(2), Dget(3)) ? 1 : 0) != 0;
 
@@ -125,7 +125,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   for (int i = 0; i < 16; ++i) acc[i] = 0.;
 
   // ======================= 1D: isoenergetic scattering =======================
-  const volatile double* vr1d = s_r1d;
+  lds_cv<double>* vr1d = lds_volatile(s_r1d);
   if (lane < 12) s_r1d[lane] = 0.;
   const double s_nux = 1.5 * T;
   double s_iso[4];
@@ -310,7 +310,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
     __syncthreads();
     const double a1e = c::PairA1E * c::PairA1E, a2e = c::PairA2E * c::PairA2E;
     const double a1x = c::PairA1X * c::PairA1X, a2x = c::PairA2X * c::PairA2X;
-    const volatile double* vbrem = s_brem;
+    lds_cv<double>* vbrem = lds_volatile(s_brem);
     for (int cc = lane; cc < L.P; cc += kWave) {
       const double eta = vin[kEtaSlot];
       const double pairT2 = c::PairPhiC * (vin[1] * vin[1]);
@@ -406,7 +406,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           float tot = 0.f;
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
-            const volatile HrChan& h = s_hc[ch];
+            lds_cv<HrChan>& h = lds_volatile(s_hc)[ch];
             float hh = static_cast<float>(h.h), gx = static_cast<float>(h.gamma_half_gb);
             float sd = expf(-pf * (static_cast<float>(h.lsd_pt) + lsdx));
             float sbase = expf(pinvf * logf(pnd + sd));
@@ -441,7 +441,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           double tot = 0.;
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
-            const volatile HrChan& h = s_hc[ch];
+            lds_cv<HrChan>& h = lds_volatile(s_hc)[ch];
             double sd = exp_bounded(-hr_p * (h.lsd_pt + lsdx));
             double sbase = exp_bounded(hr_pinv * log(pnd + sd));
             // the operands are normal and the quotients O(1e-30..1e30)

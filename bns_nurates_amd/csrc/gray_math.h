@@ -17,6 +17,19 @@
 
 namespace bns {
 
+// Wave-uniform per-point values kept in LDS and read where they are used:
+// volatile keeps the compiler from hoisting the reads out of the cell loops
+// (held in registers through them they spill), and the LDS address space
+// keeps them LDS reads.  Through a generic pointer a volatile access is left
+// in the generic address space: a flat load that waits for its completion
+// before the next instruction.
+template <class T>
+using lds_cv = const volatile __attribute__((address_space(3))) T;
+template <class T>
+__device__ __forceinline__ lds_cv<T>* lds_volatile(T* p) {
+  return (lds_cv<T>*)p;
+}
+
 // ---------------------------------------------------------------------------
 // Constants (constants.hpp)
 // ---------------------------------------------------------------------------

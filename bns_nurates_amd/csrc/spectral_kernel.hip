@@ -85,7 +85,7 @@ __host__ __device__ inline SpLds sp_lds(int n, int E) {
 // channels, at x = (omega + omega') / T (kernel_brem_HR98.hpp:33-393), given
 // the per-point scalars hp and channel constants hc
 __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
-                                           const volatile HrChan* hc) {
+                                           lds_cv<HrChan>* hc) {
   const double xc = x > c::BremMin ? x : c::BremMin;  // BremKernelS clamps x
   const double y = hp.y > c::BremMin ? hp.y : c::BremMin;
   const double snum = 2. * c::SqrtPi * exp(1.5 * log(xc + 2. - hp.emy)) *
@@ -146,7 +146,7 @@ k_spectral_f64(SpectralKernelArgs A) {
   const int64_t c_end = c_begin + A.cpw < A.nchunk ? c_begin + A.cpw : A.nchunk;
   const int64_t ld = A.ld;
   const SpLds L = sp_lds(n, E);
-  const volatile double* vin = s_in;
+  lds_cv<double>* vin = lds_volatile(s_in);
   // the LDS block: written and read in phases separated by barriers
   double* V = sm;
   double* s_x = V + L.qx;
@@ -234,13 +234,13 @@ k_spectral_f64(SpectralKernelArgs A) {
     s_brem[4] = hp.pinv;
     s_brem[5] = hp.medium;
   }
-  const volatile double* vbrem = s_brem;
+  lds_cv<double>* vbrem = lds_volatile(s_brem);
   // read where used, each chunk: loads the compiler may not hoist out of the
   // chunk loop (held in registers through it they spill)
-  const volatile double* vbpt = s_bpt;
-  const volatile double* vetasc = s_etasc;
+  lds_cv<double>* vbpt = lds_volatile(s_bpt);
+  lds_cv<double>* vetasc = lds_volatile(s_etasc);
   __syncthreads();
-  const volatile Spec* vD = s_D;
+  lds_cv<Spec>* vD = lds_volatile(s_D);
   auto Dget = [&](int s) {
     Spec r;
     r.wt = vD[s].wt;
@@ -253,7 +253,7 @@ k_spectral_f64(SpectralKernelArgs A) {
     r.lbf = vD[s].lbf;
     return r;
   };
-  const volatile double (*vFe)[6] = s_Fe;
+  lds_cv<double> (*vFe)[6] = (lds_cv<double> (*)[6])s_Fe;
   // anux with the distribution of nux: its TotalNuF values are copied
   const bool same23 = __builtin_amdgcn_readfirstlane(spec_same(Dget(2), Dget(3)) ? 1 : 0) != 0;
   const double tpair = c::PairScale * T;  // s_pair = 0.5 * 4.364 T (m1_opacities.hpp:1546)
@@ -484,7 +484,7 @@ k_spectral_f64(SpectralKernelArgs A) {
             } else {
               const double x = over_T(nu + nub);
               const HrPoint hpl{vbrem[1], vbrem[2], vbrem[3], vbrem[4], vbrem[5]};
-              sa = hr98_abs(x, hpl, s_hc);
+              sa = hr98_abs(x, hpl, lds_volatile(s_hc));
               se = sa * safe_exp(-x);
             }
           }
