@@ -566,30 +566,32 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           for (int s = 0; s < 4; ++s) out[s] = fma(b1[s], U, fma(b2[s], V, out[s]));
         } else if (xm > el * 1e-3) {
           // MezzacappaIntOneEnergy (kernel_nes.hpp:82-107)
+          const double xq = opaque(xm), yq = opaque(ym);
           double D3[6];
-          fdi_bundle<0, 2>(el - xm, D3);
+          fdi_bundle<0, 2>(el - xq, D3);
           double f0 = D3[2] - vFe[lep][2], f1 = D3[1], f2 = vFe[lep][1];
-          double f3 = D3[0] + ym * fermi_of_arg(xm - el) / 6.;
-          double f4 = vFe[lep][0] - ym * s_fd0[lep] / 6.;
-          double ix2 = 1. / (xm * xm);
+          double f3 = D3[0] + yq * fermi_of_arg(xq - el) / 6.;
+          double f4 = vFe[lep][0] - yq * s_fd0[lep] / 6.;
+          double ix2 = 1. / (xq * xq);
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
             double B1 = b1[s], B2 = b2[s];
-            double S = 2. * (B1 + B2) * f0 + (B1 * (ym + 4. * xm) + 3. * B2 * ym) * f1 +
-                       (B1 * (3. * ym - 4. * xm) + B2 * ym) * f2 +
-                       ((B1 + 6. * B2) * ym * ym * 0.2 + B1 * xm * ym + 2. * B1 * xm * xm) * f3 -
-                       ((6. * B1 + B2) * ym * ym * 0.2 - 3. * B1 * xm * ym + 2. * B1 * xm * xm) * f4;
-            out[s] = fma(pre, -sgn * ym * S * ix2, out[s]);
+            double S = 2. * (B1 + B2) * f0 + (B1 * (yq + 4. * xq) + 3. * B2 * yq) * f1 +
+                       (B1 * (3. * yq - 4. * xq) + B2 * yq) * f2 +
+                       ((B1 + 6. * B2) * yq * yq * 0.2 + B1 * xq * yq + 2. * B1 * xq * xq) * f3 -
+                       ((6. * B1 + B2) * yq * yq * 0.2 - 3. * B1 * xq * yq + 2. * B1 * xq * xq) * f4;
+            out[s] = fma(pre, -sgn * yq * S * ix2, out[s]);
           }
         } else {
           // MezzacappaIntTwoEnergies (kernel_nes.hpp:111-132)
+          const double xq = opaque(xm), yq = opaque(ym);
           double f0 = s_fd0[lep], f1 = vFe[lep][0], f2 = vFe[lep][1];
-          double ix2 = 1. / (xm * xm);
-          double pol = (11. * ym * ym - 25. * xm * ym + 20. * xm * xm) / 30.;
+          double ix2 = 1. / (xq * xq);
+          double pol = (11. * yq * yq - 25. * xq * yq + 20. * xq * xq) / 30.;
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
-            double S = (b1[s] + b2[s]) * (4. * f2 + f0 * pol) + (b1[s] - b2[s]) * (2. * xm - ym) * f1;
-            out[s] = fma(pre, ym * (w - wp) * S * ix2, out[s]);
+            double S = (b1[s] + b2[s]) * (4. * f2 + f0 * pol) + (b1[s] - b2[s]) * (2. * xq - yq) * f1;
+            out[s] = fma(pre, yq * (w - wp) * S * ix2, out[s]);
           }
         }
       }

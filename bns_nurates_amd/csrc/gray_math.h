@@ -30,6 +30,14 @@ __device__ __forceinline__ lds_cv<T>* lds_volatile(T* p) {
   return (lds_cv<T>*)p;
 }
 
+// x as a value the compiler cannot see through: work formed from it inside a
+// rarely taken branch stays in that branch instead of being hoisted to where
+// every lane pays for it.
+__device__ __forceinline__ double opaque(double x) {
+  __asm__ volatile("" : "+v"(x));
+  return x;
+}
+
 // ---------------------------------------------------------------------------
 // Constants (constants.hpp)
 // ---------------------------------------------------------------------------
