@@ -429,7 +429,8 @@ def config0_eval(ctx, steps=20, repeats=5):
     return res
 
 
-def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0, layouts=("struct", "fields")):
+def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0, layouts=("struct", "fields"),
+                  warmup_s=0.3):
     """The spectral leg: events-timed passes of the spectral kernel over the
     spectral workload (inputs resident in HBM) in each output layout -- one
     SpectralOpacities record per (point, energy) (bnsamd_spectral_opacities_aos)
@@ -460,12 +461,21 @@ def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0, layouts=("struct",
             ctx.compute_spectral_opacities(dsp, dnu, W.FLAGS_ALL, prm, stim_abs=0, out=out,
                                            layout=layout)
 
-        for _ in range(warmup):
+        # a launch is ~5 ms: the warm-up runs for warmup_s of wall time (at
+        # least `warmup` launches) so that the clocks have left their idle
+        # state, as in a loop that keeps the device busy
+        t0 = time.perf_counter()
+        nw = 0
+        while nw < warmup or time.perf_counter() - t0 < warmup_s:
             run()
+            nw += 1
+            if nw % 8 == 0:
+                torch.cuda.synchronize()
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(steps)]
-        log(f"spectral ({layout} layout): timing {steps} steps ({npts} points x {nnu} energies)")
+        log(f"spectral ({layout} layout): {nw} warm-up launches, timing {steps} steps "
+            f"({npts} points x {nnu} energies)")
         t0 = time.perf_counter()
         for a, b in ev:
             a.record()
@@ -480,7 +490,7 @@ def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0, layouts=("struct",
         excess = float(O.parity_excess(o[:, :, idx], ref, mag).max())
         achieved = SPECTRAL_FLOPS_PER_EVAL * nev / (kern_ms * 1e-3) / 1e12
         legs[layout] = {"value": nev * steps / wall, "kernel_ms": kern_ms, "achieved": achieved,
-                        "bar_excess_max": excess}
+                        "bar_excess_max": excess, "warmup_launches": nw}
         log(f"spectral ({layout}): {legs[layout]['value']:.4g} evals/s, kernel {kern_ms:.2f} ms, "
             f"{achieved:.2f} TFLOP/s, parity excess {excess:.3g}")
         del out
@@ -492,7 +502,8 @@ def spectral_eval(ctx, steps, warmup, cpu=True, target_s=8.0, layouts=("struct",
            "layout": {"struct": "SpectralOpacities[npts][nnu] (bnsamd_spectral_opacities_aos)",
                       "fields": "[16][nnu][npts] (bnsamd_spectral_opacities)"}[layouts[0]],
            "value": main["value"], "unit": "(point, energy) evaluations/s",
-           "steps": steps, "kernel_ms": main["kernel_ms"],
+           "steps": steps, "warmup_launches": main["warmup_launches"],
+           "kernel_ms": main["kernel_ms"],
            "roofline": {"bound": "fp64 vector ALU", "achieved": main["achieved"],
                         "peak": PEAK_FP64_VECTOR_TFLOPS, "unit": "TFLOP/s",
                         "frac": main["achieved"] / PEAK_FP64_VECTOR_TFLOPS,
@@ -821,7 +832,7 @@ def main():
     if not args.no_spectral:
         lay = {"both": ("struct", "fields"), "struct": ("struct",),
                "fields": ("fields",)}[args.spectral_layout]
-        line["spectral"] = spectral_eval(ctx, max(3, args.steps // 2), 1,
+        line["spectral"] = spectral_eval(ctx, max(10, 2 * args.steps), max(1, args.warmup),
                                          cpu=not args.no_cpu, layouts=lay)
     if not args.no_serving:
         line["serving"] = serving_sweep()
