@@ -18,6 +18,7 @@
 // same product.)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -72,13 +73,15 @@ __device__ inline float silu_grad(float z) {
 // C[m][n] (+)= sum_k A(m,k) B(k,n), A(m,k) = A[m sam + k sak],
 // B(k,n) = B[k sbk + n sbn] (BF16), C FP32 with leading dimension ldc.
 // 64 x 64 tile per workgroup, 4 waves of 32 x 32, K in steps of 32; gridDim.z
-// splits K into chunks of kchunk (ATOMIC: the chunks add into C).
-template <bool ATOMIC>
+// splits K into chunks of kchunk (SPLIT: chunk z writes its partial product
+// to C + z csplit, summed in chunk order by k_sum_splits, so that the result
+// does not depend on the order in which workgroups finish).
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t* __restrict__ A,
                                               int64_t sam, int64_t sak,
                                               const uint16_t* __restrict__ B, int64_t sbk,
                                               int64_t sbn, float* __restrict__ C, int64_t ldc,
-                                              int kchunk) {
+                                              int kchunk, int64_t csplit) {
   __shared__ __attribute__((aligned(16))) uint16_t As[64][40];  // [m][k]
   __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];  // [n][k]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -125,10 +128,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v, col = n0 + wn + j * 16 + (lane & 15);
-        if (row < M && col < N) {
-          if (ATOMIC) atomicAdd(&C[row * ldc + col], acc[i][j][v]);
-          else C[row * ldc + col] = acc[i][j][v];
-        }
+        if (row < M && col < N) (SPLIT ? C + blockIdx.z * csplit : C)[row * ldc + col] = acc[i][j][v];
       }
 }
 
@@ -139,11 +139,12 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
 // each operand per k step (transposed into the [m][k] / [n][k] LDS tiles
 // where the contiguous dimension is m or n), and the next step's pieces are
 // loaded while the current step multiplies.
-template <bool ATOMIC, bool AK, bool BK>
+template <bool SPLIT, bool AK, bool BK>
 __global__ __launch_bounds__(256) void k_gemm_v(int M, int N, int K,
                                                 const uint16_t* __restrict__ A, int64_t lda,
                                                 const uint16_t* __restrict__ B, int64_t ldb,
-                                                float* __restrict__ C, int64_t ldc, int kchunk) {
+                                                float* __restrict__ C, int64_t ldc, int kchunk,
+                                                int64_t csplit) {
   __shared__ __attribute__((aligned(16))) uint16_t As[64][40];  // [m][k]
   __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];  // [n][k]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -212,16 +213,45 @@ __global__ __launch_bounds__(256) void k_gemm_v(int M, int N, int K,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v, col = n0 + wn + j * 16 + (lane & 15);
-        if (row < M && col < N) {
-          if (ATOMIC) atomicAdd(&C[int64_t(row) * ldc + col], acc[i][j][v]);
-          else C[int64_t(row) * ldc + col] = acc[i][j][v];
-        }
+        if (row < M && col < N)
+          (SPLIT ? C + blockIdx.z * csplit : C)[int64_t(row) * ldc + col] = acc[i][j][v];
       }
 }
 
 // batch rows per split-K chunk of the weight-gradient products (256: 16
-// chunks at batch 4096; 512 measured 12% slower per step, 128 no faster)
+// chunks at batch 4096; 512 measured 12% slower per step, 128 no faster), at
+// most kMaxSplits chunks (the partial products' workspace)
 constexpr int kDwRows = 256;
+constexpr int kMaxSplits = 16;
+constexpr int kMaxSplitElems = kHidden * kHidden;  // largest M x N of a weight gradient
+
+// C[i] += sum over z of part[z n + i], in z order
+__global__ void k_sum_splits(int64_t n, int z, const float* __restrict__ part,
+                             float* __restrict__ C) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = part[i];
+  for (int k = 1; k < z; ++k) s += part[int64_t(k) * n + i];
+  C[i] += s;
+}
+
+// db[c] += sum over b of part[b cols + c] for one column per workgroup: each
+// thread sums a fixed stride of the partials, then a fixed tree
+__global__ __launch_bounds__(256) void k_sum_columns(int nblk, int cols,
+                                                     const float* __restrict__ part,
+                                                     float* __restrict__ db) {
+  __shared__ float red[256];
+  const int c = blockIdx.x, t = threadIdx.x;
+  float s = 0.f;
+  for (int b = t; b < nblk; b += 256) s += part[int64_t(b) * cols + c];
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) db[c] += red[0];
+}
 // rows per workgroup of the element-wise training kernels (one thread per
 // column): 512 workgroups at batch 4096
 constexpr int kEwRows = 8;
@@ -240,11 +270,12 @@ __global__ void k_bias_act(int rows, int cols, float* __restrict__ Z, const floa
   }
 }
 
-// dZ = dH act'(Z) as BF16, db += column sums of dZ (one thread per column,
-// kEwRows rows per workgroup)
+// dZ = dH act'(Z) as BF16, and this workgroup's column sums of dZ into
+// part[blockIdx.x cols + c] (one thread per column, kEwRows rows per
+// workgroup; k_sum_columns adds them to db)
 __global__ void k_act_bwd(int rows, int cols, const float* __restrict__ dH,
                           const float* __restrict__ Z, uint16_t* __restrict__ dZ,
-                          float* __restrict__ db) {
+                          float* __restrict__ part) {
   int c = threadIdx.x, r0 = blockIdx.x * kEwRows;
   if (c >= cols) return;
   float s = 0.f;
@@ -254,16 +285,18 @@ __global__ void k_act_bwd(int rows, int cols, const float* __restrict__ dH,
     dZ[i] = f2bf(d);
     s += d;
   }
-  atomicAdd(&db[c], s);
+  part[int64_t(blockIdx.x) * cols + c] = s;
 }
 
 // last layer: y = Z (bias added), e = y - t on the first nout columns (0
-// where the target is NaN: an output missing for that row); loss += sum e^2,
-// dZ = 2 e / (rows nout) as BF16, db += column sums
+// where the target is NaN: an output missing for that row); dZ = 2 e /
+// (rows nout) as BF16; this workgroup's column sums of dZ into
+// part[blockIdx.x cols + c] and its sum of e^2 into lpart[blockIdx.x]
+// (summed in workgroup order by k_sum_columns and k_sum_loss)
 __global__ void k_mse_grad(int rows, int cols, int nout, const float* __restrict__ Z,
                            const float* __restrict__ T, uint16_t* __restrict__ dZ,
-                           float* __restrict__ db, float* __restrict__ loss) {
-  __shared__ float red[256];
+                           float* __restrict__ part, float* __restrict__ lpart) {
+  __shared__ float red[256], cs[256];
   int c = threadIdx.x % cols, rl = threadIdx.x / cols, rstep = blockDim.x / cols;
   int r0 = blockIdx.x * 256;
   float s = 0.f, l2 = 0.f;
@@ -280,13 +313,26 @@ __global__ void k_mse_grad(int rows, int cols, int nout, const float* __restrict
     }
   }
   red[threadIdx.x] = l2;
+  cs[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(loss, red[0]);
-  if (rl < rstep) atomicAdd(&db[c], s);
+  if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+  if (threadIdx.x < cols) {
+    float t = 0.f;
+    for (int r = 0; r < rstep; ++r) t += cs[r * cols + threadIdx.x];
+    part[int64_t(blockIdx.x) * cols + threadIdx.x] = t;
+  }
+}
+
+// loss[0] += sum of the n workgroup sums, in order (one thread)
+__global__ void k_sum_loss(int n, const float* __restrict__ lpart, float* __restrict__ loss) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float s = 0.f;
+  for (int i = 0; i < n; ++i) s += lpart[i];
+  loss[0] += s;
 }
 
 // Adam step over the whole parameter blob; refreshes the BF16 copy
@@ -669,6 +715,12 @@ struct Work {
   uint16_t* dZ[kLayers] = {};
   float* dH = nullptr;
   float* loss = nullptr;
+  // per-workgroup partial sums, added in a fixed order so that training is
+  // reproducible: split-K weight-gradient products, bias-gradient column
+  // sums, loss sums
+  float* part = nullptr;
+  float* colpart = nullptr;
+  float* lpart = nullptr;
 };
 }  // namespace mlp
 }  // namespace bns
@@ -710,30 +762,42 @@ int mlp_fail(bnsamd_mlp* m, int code, const std::string& msg) {
   return code;
 }
 
-template <bool ATOMIC>
+template <bool SPLIT>
 void gemm_v(hipStream_t s, dim3 grid, bool ak, bool bk, int M, int N, int K, const uint16_t* A,
-            int64_t lda, const uint16_t* B, int64_t ldb, float* C, int64_t ldc, int kchunk) {
+            int64_t lda, const uint16_t* B, int64_t ldb, float* C, int64_t ldc, int kchunk,
+            int64_t csplit) {
   const dim3 block(256);
   if (ak && bk)
-    hipLaunchKernelGGL((k_gemm_v<ATOMIC, true, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb,
-                       C, ldc, kchunk);
+    hipLaunchKernelGGL((k_gemm_v<SPLIT, true, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb,
+                       C, ldc, kchunk, csplit);
   else if (ak)
-    hipLaunchKernelGGL((k_gemm_v<ATOMIC, true, false>), grid, block, 0, s, M, N, K, A, lda, B,
-                       ldb, C, ldc, kchunk);
+    hipLaunchKernelGGL((k_gemm_v<SPLIT, true, false>), grid, block, 0, s, M, N, K, A, lda, B,
+                       ldb, C, ldc, kchunk, csplit);
   else if (bk)
-    hipLaunchKernelGGL((k_gemm_v<ATOMIC, false, true>), grid, block, 0, s, M, N, K, A, lda, B,
-                       ldb, C, ldc, kchunk);
+    hipLaunchKernelGGL((k_gemm_v<SPLIT, false, true>), grid, block, 0, s, M, N, K, A, lda, B,
+                       ldb, C, ldc, kchunk, csplit);
   else
-    hipLaunchKernelGGL((k_gemm_v<ATOMIC, false, false>), grid, block, 0, s, M, N, K, A, lda, B,
-                       ldb, C, ldc, kchunk);
+    hipLaunchKernelGGL((k_gemm_v<SPLIT, false, false>), grid, block, 0, s, M, N, K, A, lda, B,
+                       ldb, C, ldc, kchunk, csplit);
 }
 
+// C = A B, or with splitk > 1 C += A B as splitk partial products over
+// chunks of K written to part (room for kMaxSplits x M x N floats; C then
+// M x N contiguous, ldc = N) and summed in chunk order
 void gemm(hipStream_t s, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak,
-          const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, int splitk) {
+          const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, int splitk,
+          float* part = nullptr) {
   int kchunk = (K + splitk - 1) / splitk;
   kchunk = (kchunk + 31) / 32 * 32;
   int z = (K + kchunk - 1) / kchunk;
   dim3 grid((N + 63) / 64, (M + 63) / 64, z), block(256);
+  const int64_t mn = int64_t(M) * N;
+  float* const Cz = z > 1 ? part : C;  // where the kernel writes
+  auto finish = [&]() {
+    if (z > 1)
+      hipLaunchKernelGGL(k_sum_splits, dim3(unsigned((mn + 255) / 256)), dim3(256), 0, s, mn, z,
+                         part, C);
+  };
   // 16-byte pieces: the contiguous dimension of each operand has stride 1,
   // its extent and the other stride are multiples of 8, the bases are
   // 16-byte aligned
@@ -744,16 +808,18 @@ void gemm(hipStream_t s, int M, int N, int K, const uint16_t* A, int64_t sam, in
                    ldb % 8 == 0 && (ak ? K % 8 == 0 : M % 8 == 0) &&
                    (bk ? K % 8 == 0 : N % 8 == 0);
   if (vec) {
-    if (z > 1) gemm_v<true>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, C, ldc, kchunk);
-    else gemm_v<false>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, C, ldc, kchunk);
+    if (z > 1) gemm_v<true>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, Cz, ldc, kchunk, mn);
+    else gemm_v<false>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, Cz, ldc, kchunk, 0);
+    finish();
     return;
   }
   if (z > 1)
-    hipLaunchKernelGGL(k_gemm<true>, grid, block, 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
-                       kchunk);
+    hipLaunchKernelGGL(k_gemm<true>, grid, block, 0, s, M, N, K, A, sam, sak, B, sbk, sbn, Cz, ldc,
+                       kchunk, mn);
   else
-    hipLaunchKernelGGL(k_gemm<false>, grid, block, 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C,
-                       ldc, kchunk);
+    hipLaunchKernelGGL(k_gemm<false>, grid, block, 0, s, M, N, K, A, sam, sak, B, sbk, sbn, Cz,
+                       ldc, kchunk, int64_t(0));
+  finish();
 }
 
 void fill(hipStream_t s, int64_t n, float* p, float v) {
@@ -780,19 +846,26 @@ void backward(bnsamd_mlp* m, Work& W, int e, int B) {
   hipStream_t s = W.stream;
   fill(s, kParams, m->g[e], 0.f);
   const int L = kLayers - 1;
-  hipLaunchKernelGGL(k_mse_grad, dim3((B + 255) / 256), dim3(256), 0, s, B, kNL, kOut, W.Z[L],
-                     W.Tb, W.dZ[L], m->g[e] + kBOff[L], W.loss);
+  const int nmse = (B + 255) / 256, nbwd = (B + kEwRows - 1) / kEwRows;
+  hipLaunchKernelGGL(k_mse_grad, dim3(nmse), dim3(256), 0, s, B, kNL, kOut, W.Z[L], W.Tb,
+                     W.dZ[L], W.colpart, W.lpart);
+  hipLaunchKernelGGL(k_sum_columns, dim3(kNL), dim3(256), 0, s, nmse, kNL, W.colpart,
+                     m->g[e] + kBOff[L]);
+  hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, s, nmse, W.lpart, W.loss);
   for (int l = L; l >= 0; --l) {
     const int K = kLayerK[l], N = kLayerN[l];
     const uint16_t* Hin = l ? W.H[l - 1] : W.Xb;
-    // dW = H_in^T dZ (reduction over the batch, split into kDwRows-row chunks)
+    // dW = H_in^T dZ (reduction over the batch, split into chunks of at
+    // least kDwRows rows)
     gemm(s, K, N, B, Hin, 1, K, W.dZ[l], N, 1, m->g[e] + kWOff[l], N,
-         (B + kDwRows - 1) / kDwRows);
+         std::min((B + kDwRows - 1) / kDwRows, kMaxSplits), W.part);
     if (l) {
       // dH_in = dZ W^T, then through the SiLU of layer l - 1
       gemm(s, B, K, N, W.dZ[l], N, 1, m->wb[e] + kWOff[l], 1, N, W.dH, K, 1);
-      hipLaunchKernelGGL(k_act_bwd, dim3((B + kEwRows - 1) / kEwRows), dim3(256), 0, s, B, K,
-                         W.dH, W.Z[l - 1], W.dZ[l - 1], m->g[e] + kBOff[l - 1]);
+      hipLaunchKernelGGL(k_act_bwd, dim3(nbwd), dim3(256), 0, s, B, K, W.dH, W.Z[l - 1],
+                         W.dZ[l - 1], W.colpart);
+      hipLaunchKernelGGL(k_sum_columns, dim3(K), dim3(256), 0, s, nbwd, K, W.colpart,
+                         m->g[e] + kBOff[l - 1]);
     }
   }
   hipLaunchKernelGGL(k_tie_grad_l0, dim3(1), dim3(kHidden), 0, s, m->g[e]);
@@ -824,6 +897,10 @@ bool work_alloc(Work& W, int maxb) {
   }
   alloc(reinterpret_cast<void**>(&W.dH), sizeof(float) * B * kHidden);
   alloc(reinterpret_cast<void**>(&W.loss), sizeof(float));
+  alloc(reinterpret_cast<void**>(&W.part), sizeof(float) * kMaxSplits * kMaxSplitElems);
+  alloc(reinterpret_cast<void**>(&W.colpart),
+        sizeof(float) * ((B + kEwRows - 1) / kEwRows) * kHidden);
+  alloc(reinterpret_cast<void**>(&W.lpart), sizeof(float) * ((B + 255) / 256));
   return ok;
 }
 
@@ -841,6 +918,9 @@ void work_free(Work& W) {
   }
   fr(W.dH);
   fr(W.loss);
+  fr(W.part);
+  fr(W.colpart);
+  fr(W.lpart);
   if (W.own_stream && W.stream) (void)hipStreamDestroy(W.stream);
   W = Work{};
 }

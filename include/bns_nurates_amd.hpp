@@ -326,6 +326,22 @@ class Context {
                                     stimulated ? 1 : 0, d_out));
   }
 
+  // The same into a device array of npts * nnu SpectralOpacities records,
+  // d_out[i * nnu + e] for point i at energy e (bnsamd_spectral_opacities_aos).
+  void ComputeSpectralOpacitiesDevice(int64_t npts, int64_t ld, const double* d_pts,
+                                      const double* d_distr, int64_t nnu, const double* d_nu,
+                                      int64_t nu_ld, const OpacityFlags& flags,
+                                      const OpacityParams& params, bool stimulated,
+                                      SpectralOpacities* d_out) {
+    static_assert(sizeof(SpectralOpacities) == BNSAMD_SP_NFIELDS * sizeof(double),
+                  "SpectralOpacities must be 16 packed doubles");
+    bnsamd_flags f = detail::to_c(flags);
+    bnsamd_params p = detail::to_c(params);
+    check(bnsamd_spectral_opacities_aos(ctx_, npts, ld, d_pts, d_distr, nnu, d_nu, nu_ld, &f,
+                                        &p, stimulated ? 1 : 0,
+                                        reinterpret_cast<double*>(d_out)));
+  }
+
   // Host batch in the reference's types: entry i is point pts[i] at energy
   // nu[i] [MeV], with its own distr_pars, opacity_flags and opacity_pars, as
   // the reference's per-(energy, point) calls (tests/tests.hpp:1174-1176).
