@@ -398,8 +398,7 @@ BNS_HD double fdi_reflect(double fd, double y) {
 template <int K>
 struct FdiUnroll {
   template <int KHI>
-  BNS_HD static void eval(bool ebr, bool refl, double ex, double A, double t, double s, double y,
-                          double* F) {
+  BNS_HD static void eval(bool ebr, double ex, double A, double t, double s, double* F) {
     double p, q;
     if (ebr) {
       fdi_exp_pq<K>(t, p, q);
@@ -413,8 +412,8 @@ struct FdiUnroll {
     // all orders would keep every (p, q) pair live at once
     asm volatile("" : "+v"(fd));
 #endif
-    F[K] = refl ? fdi_reflect<K>(fd, y) : fd;
-    if constexpr (K < KHI) FdiUnroll<K + 1>::template eval<KHI>(ebr, refl, ex, A, t, s, y, F);
+    F[K] = fd;
+    if constexpr (K < KHI) FdiUnroll<K + 1>::template eval<KHI>(ebr, ex, A, t, s, F);
   }
 };
 
@@ -427,10 +426,23 @@ constexpr double kFdiDeep = -40.;
 template <int K>
 struct FdiDeepUnroll {
   template <int KHI>
-  BNS_HD static void eval(bool refl, double ex, double y, double* F) {
-    const double fd = ex * Fdi<K>::a0;
-    F[K] = refl ? fdi_reflect<K>(fd, y) : fd;
-    if constexpr (K < KHI) FdiDeepUnroll<K + 1>::template eval<KHI>(refl, ex, y, F);
+  BNS_HD static void eval(double ex, double* F) {
+    F[K] = ex * Fdi<K>::a0;
+    if constexpr (K < KHI) FdiDeepUnroll<K + 1>::template eval<KHI>(ex, F);
+  }
+};
+
+// The reflection to y > 0 of every order of a bundle, under one branch: the
+// lanes that do not reflect are masked off rather than given a select per
+// order, and a wavefront none of whose lanes reflects skips the polynomials
+// (for eta >= 0 every positron argument of the NES kernels is <= 0, for
+// eta < 0 every electron one).
+template <int K>
+struct FdiReflect {
+  template <int KHI>
+  BNS_HD static void apply(double y, double* F) {
+    F[K] = fdi_reflect<K>(F[K], y);
+    if constexpr (K < KHI) FdiReflect<K + 1>::template apply<KHI>(y, F);
   }
 };
 
@@ -449,7 +461,8 @@ BNS_HD void fdi_bundle(double y, double* F) {
   double x = -fabs(y);
 #if defined(__HIP_DEVICE_COMPILE__)
   if (DEEP && __all(x < kFdiDeep)) {
-    FdiDeepUnroll<KLO>::template eval<KHI>(y > 0., exp_bounded(fmax(x, -746.)), y, F);
+    FdiDeepUnroll<KLO>::template eval<KHI>(exp_bounded(fmax(x, -746.)), F);
+    if (y > 0.) FdiReflect<KLO>::template apply<KHI>(y, F);
     return;
   }
 #endif
@@ -465,7 +478,8 @@ BNS_HD void fdi_bundle(double y, double* F) {
     s = rbr ? -0.5 * x : 0.;
     t = 1. - s;
   }
-  FdiUnroll<KLO>::template eval<KHI>(ebr, y > 0., ex, A, t, s, y, F);
+  FdiUnroll<KLO>::template eval<KHI>(ebr, ex, A, t, s, F);
+  if (y > 0.) FdiReflect<KLO>::template apply<KHI>(y, F);
 }
 
 template <int K>
