@@ -254,7 +254,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
         int a = task - 2 * N;
         double nu = node_nu(a);
         s_y[a] = div_by_recip(nu, T, vin[kInvT]);
-        double lw = MIX ? 0. : log(nu);
+        double lw = MIX ? 0. : log_bounded(nu);
         float lw32 = MIX ? logf(static_cast<float>(nu)) : 0.f;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -427,12 +427,12 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           // the exponents below are logarithms of O(1e-30..1e30) quantities
           // scaled by O(1) powers: the bounded exponential (within an ulp of
           // e^x) without the library's range handling
-          double snum = 2. * c::SqrtPi * exp_bounded(1.5 * log(x + 2. - hr_emy)) *
+          double snum = 2. * c::SqrtPi * exp_bounded(1.5 * log_bounded(x + 2. - hr_emy)) *
                         (x * x + 2. * x * y + 5. * (y * y) / 3. + 1.);
           double bb = c::Pi2OneEighth + x + y;
           double b2 = bb * bb;
           double snd = div_normal(snum, c::SqrtPi + b2 * b2);
-          double lsnd = log(snd), lsdx = cr[CL_LSD];
+          double lsnd = log_bounded(snd), lsdx = cr[CL_LSD];
           double pnd = exp_bounded(-hr_p * lsnd);
           double px11 = cr[CL_X11], px093 = cr[CL_X093];
           double px12 = cr[CL_X12], px28 = cr[CL_X28];
@@ -443,7 +443,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           for (int ch = 0; ch < 3; ++ch) {
             lds_cv<HrChan>& h = lds_volatile(s_hc)[ch];
             double sd = exp_bounded(-hr_p * (h.lsd_pt + lsdx));
-            double sbase = exp_bounded(hr_pinv * log(pnd + sd));
+            double sbase = exp_bounded(hr_pinv * log_bounded(pnd + sd));
             // the operands are normal and the quotients O(1e-30..1e30)
             double fb = 1. + div_normal(1., fx * h.f_pt);
             double gbm = 1. - px11 * h.g0;

@@ -174,6 +174,44 @@ BNS_HD double div_by_recip(double a, double b, double rb) {
   return fma(r, rb, q);
 }
 
+// log(x) for positive, finite, normal x (callers guarantee it; the HR98
+// bremsstrahlung logarithms and neutrino energies), without the library's
+// special-case handling: x = 2^k m with m in [sqrt(1/2), sqrt(2)), f = m - 1
+// (exact), s = f / (2 + f), log m = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R))
+// with R = 2 sum_{j=1..10} s^{2j} / (2j + 1) (the series' first omitted term
+// is below 1e-18 relative at |s| <= 0.1716), and k ln 2 with ln 2 split so
+// that k ln2_hi is exact.  The same operations on the host and the device
+// (contraction off), so that the host check covers the device code: within
+// one ulp of log(x) (tests/cpp/log_bounded_check.cpp).
+BNS_HD double log_bounded(double x) {
+#pragma clang fp contract(off)
+  int k;
+  double m = frexp(x, &k);  // [0.5, 1)
+  if (m < 0x1.6a09e667f3bcdp-1) {
+    m += m;
+    --k;
+  }
+  const double f = m - 1.;
+  const double s = div_normal(f, 2. + f);
+  const double z = s * s;
+  double r = 0x1.8618618618618p-4;  // 2/21
+  r = fma(z, r, 0x1.af286bca1af28p-4);  // 2/19
+  r = fma(z, r, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
+  r = fma(z, r, 0x1.1111111111111p-3);  // 2/15
+  r = fma(z, r, 0x1.3b13b13b13b14p-3);  // 2/13
+  r = fma(z, r, 0x1.745d1745d1746p-3);  // 2/11
+  r = fma(z, r, 0x1.c71c71c71c71cp-3);  // 2/9
+  r = fma(z, r, 0x1.2492492492492p-2);  // 2/7
+  r = fma(z, r, 0x1.999999999999ap-2);  // 2/5
+  r = fma(z, r, 0x1.5555555555555p-1);  // 2/3
+  r = r * z;
+  const double hfsq = 0.5 * (f * f);
+  const double dk = static_cast<double>(k);
+  const double ln2_hi = 0x1.62e42ff000000p-1, ln2_lo = -0x1.718432a1b0e26p-35;
+  const double corr = fma(s, hfsq + r, dk * ln2_lo);
+  return fma(dk, ln2_hi, f - (hfsq - corr));
+}
+
 // FermiDistr (functions.hpp:4376-4395) given the already-formed argument
 // (e - mu) / T.  Written without a branch: 1/(e^a + 1) = e^-a/(e^-a + 1);
 // the denominator lies in [1, 2] and the numerator in [e^-700, 1].

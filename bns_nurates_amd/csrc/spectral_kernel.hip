@@ -86,19 +86,23 @@ __host__ __device__ inline SpLds sp_lds(int n, int E) {
 // the per-point scalars hp and channel constants hc
 __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
                                            lds_cv<HrChan>* hc) {
+  // the logarithms' arguments are positive and normal and the exponents
+  // logarithms of O(1e-30..1e30) quantities scaled by O(1) powers: the
+  // bounded forms, without the library's special-case handling
   const double xc = x > c::BremMin ? x : c::BremMin;  // BremKernelS clamps x
   const double y = hp.y > c::BremMin ? hp.y : c::BremMin;
-  const double snum = 2. * c::SqrtPi * exp(1.5 * log(xc + 2. - hp.emy)) *
+  const double snum = 2. * c::SqrtPi * exp_bounded(1.5 * log_bounded(xc + 2. - hp.emy)) *
                       (xc * xc + 2. * xc * y + 5. * (y * y) / 3. + 1.);
   const double bb = c::Pi2OneEighth + xc + y;
   const double b2 = bb * bb;
   const double snd = div_normal(snum, c::SqrtPi + b2 * b2);
-  const double lx = log(xc);
-  const double px11 = exp(1.1 * lx), pxm4 = exp(-4. * lx), px093 = exp(0.93 * lx);
-  const double px12 = exp(1.2 * lx), px28 = exp(2.8 * lx);
-  const double lsdx =
-      log((xc * xc + c::FourPiSquared) * xc / (c::FourPiSquared * (1. - safe_exp(-xc))));
-  const double pnd = exp(-hp.p * log(snd));
+  const double lx = log_bounded(xc);
+  const double px11 = exp_bounded(1.1 * lx), pxm4 = exp_bounded(-4. * lx);
+  const double px093 = exp_bounded(0.93 * lx);
+  const double px12 = exp_bounded(1.2 * lx), px28 = exp_bounded(2.8 * lx);
+  const double lsdx = log_bounded((xc * xc + c::FourPiSquared) * xc /
+                                  (c::FourPiSquared * (1. - safe_exp(-xc))));
+  const double pnd = exp_bounded(-hp.p * log_bounded(snd));
   const double xm = xc - 1.2;
   const double fx = 3. + xm * xm + pxm4;
   double tot = 0.;
@@ -106,8 +110,8 @@ __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
   for (int ch = 0; ch < 3; ++ch) {
     const double lsd_pt = hc[ch].lsd_pt, f_pt = hc[ch].f_pt, g0 = hc[ch].g0, h = hc[ch].h;
     const double gx = hc[ch].gamma_half_gb, got = hc[ch].gamma_over_T, wt = hc[ch].weight;
-    double sd = exp(-hp.p * (lsd_pt + lsdx));
-    double sbase = exp(hp.pinv * log(pnd + sd));
+    double sd = exp_bounded(-hp.p * (lsd_pt + lsdx));
+    double sbase = exp_bounded(hp.pinv * log_bounded(pnd + sd));
     double fb = 1. + div_normal(1., fx * f_pt);
     double gbm = 1. - px11 * g0;
     double cb = div_normal(div_normal(1.1 * px11 * h, 2.3 + h * px093 + 0.0001 * px12) * 30.,
@@ -282,7 +286,7 @@ k_spectral_f64(SpectralKernelArgs A) {
         const double nub = pair_node(a);
         V[L.nub + a] = nub;
         V[L.z + a] = over_T(nub);
-        const double lw = log(nub);
+        const double lw = log_bounded(nub);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const double g = nuf(nub, lw, Dget(s));
@@ -328,7 +332,11 @@ k_spectral_f64(SpectralKernelArgs A) {
         // TotalNuF(nu) of species s (m1_opacities.hpp:1533-1536)
         const int e = t >> 2, sp = t & 3;
         const double nu = energy(e);
-        const double lw = log(nu);
+        // the caller's energies: the library logarithm for a zero, negative
+        // or non-finite one
+        double lw;
+        if (nu >= 0x1p-1022 && nu <= 0x1.fffffffffffffp+1023) lw = log_bounded(nu);
+        else lw = log(nu);
         if (sp == 0) {
           V[L.nu + e] = nu;
           V[L.y + e] = over_T(nu);
