@@ -745,12 +745,13 @@ void launch_instance(const GrayKernelArgs& a, dim3 grid, hipStream_t stream) {
   // the 128^3-grid workload's order, n = 16, has its own instance (0.6%
   // faster: its table offsets and loop bounds are constants)
   const bool fixed = a.L.n1 == 16 && a.L.n2 == 16;
-  // GP19 bremsstrahlung over a table: its own instances, 2 waves per SIMD
+  // GP19 bremsstrahlung over a table: its own instances, 3 waves per SIMD
+  // (163 VGPRs)
   if (a.gp19 && a.params.brem_impl == 2 && a.flags.use_brem == 1) {
     if constexpr (MIX)
-      hipLaunchKernelGGL((k_gray_m1_mixed<2, STIM, BLOCK, 0, true>), grid, dim3(64), 0, stream, a);
+      hipLaunchKernelGGL((k_gray_m1_mixed<3, STIM, BLOCK, 0, true>), grid, dim3(64), 0, stream, a);
     else
-      hipLaunchKernelGGL((k_gray_m1_f64<2, STIM, BLOCK, 0, true>), grid, dim3(64), 0, stream, a);
+      hipLaunchKernelGGL((k_gray_m1_f64<3, STIM, BLOCK, 0, true>), grid, dim3(64), 0, stream, a);
     return;
   }
   if constexpr (MIX) {
