@@ -49,7 +49,7 @@ TRAFFIC_BYTES_PER_POINT = (346467.3 + 327680) * 1024 / 2097152
 ALGORITHMIC_BYTES_PER_POINT = 8 * (21 + 20)
 # hardware FP64 FLOPs per point, (ADD + MUL + TRANS + 2 FMA) x 64 lanes from
 # the SQ_INSTS_VALU_*_F64 counters over the 2,097,152-point launch
-HW_FLOPS_PER_POINT = 1551257.9
+HW_FLOPS_PER_POINT = 1541192.7
 
 
 # The spectral path (ComputeSpectralOpacitiesNotStimulatedAbs): 32^3 points
