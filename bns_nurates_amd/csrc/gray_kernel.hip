@@ -80,13 +80,13 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
                                         vin[21 + 12 + s], vin[21 + 16 + s], vin[21 + 20 + s],
                                         vin[21 + 24 + s])
                      : spec_from_m1(vin[9 + s], vin[13 + s], vin[17 + s], A.gamma36);
-    d.lbf = MIX ? log(d.bf) : 0.;
+    d.lbf = MIX ? log_pos(d.bf) : 0.;
     s_D[s] = d;
   }
   // 1 / T: energies / T are formed as div_by_recip(nu, T, 1 / T); log(T):
   // the logarithms of the NEPS energies are log(T) + log(nu / T)
   if (lane == 63) s_in[kInvT] = 1. / T;
-  if (lane == 62) s_in[kLogT] = log(T);
+  if (lane == 62) s_in[kLogT] = log_pos(T);
   if (lane == 61) s_in[kEtaSlot] = mu_e / T;
 
   // ---- per-point constants --------------------------------------------------
@@ -283,7 +283,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       const double nb = vin[0], yp = vin[2], yn = vin[3];
       double rho = nb * c::Mb;
       double q = 2.0778e+02 * 0.5 * c::MeV * (yn * yn + yp * yp + (28. / 3.) * yn * yp) *
-                 (rho * rho) * exp(5.5 * log(T));
+                 (rho * rho) * exp_bounded(5.5 * log_pos(T));
       double T2 = T * T, T7 = T2 * T2 * T2 * T;
       brt_pre = c::HC6FourPiSq * c::BremC4BRT06 * (q / T7);
     } else if (do_brem && A.params.brem_impl == 0) {

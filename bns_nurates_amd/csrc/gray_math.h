@@ -212,6 +212,17 @@ BNS_HD double log_bounded(double x) {
   return fma(dk, ln2_hi, f - (hfsq - corr));
 }
 
+// log(x) for any x: the bounded form where x is positive, finite and normal,
+// the library logarithm otherwise (zero, subnormals, inf and NaN keep its
+// special values; a wavefront none of whose lanes needs it skips it)
+BNS_HD double log_pos(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023) ? log_bounded(x) : log(x);
+#else
+  return std::log(x);
+#endif
+}
+
 // FermiDistr (functions.hpp:4376-4395) given the already-formed argument
 // (e - mu) / T.  Written without a branch: 1/(e^a + 1) = e^-a/(e^-a + 1);
 // the denominator lies in [1, 2] and the numerator in [e^-700, 1].

@@ -97,7 +97,7 @@ __device__ __forceinline__ GOmg dist(double w, const Spec& d) {
   if constexpr (MIX) {
     o = nuf_mixed(w, logf(static_cast<float>(w)), d);
   } else {
-    o.g = nuf(w, log(w), d);
+    o.g = nuf(w, log_pos(w), d);
     o.omg = 1. - o.g;
   }
   return o;
@@ -253,7 +253,7 @@ __device__ __forceinline__ void beta_em_ab(double nu, int s, const BetaPoint& bp
 __device__ __forceinline__ double eta_nn_sc(double nb, double T, double yN) {
   double nN = yN * nb;
   if (nN <= 0.) return 0.;
-  double eF = c::IsoEF * exp((2. / 3.) * log(nN));
+  double eF = c::IsoEF * exp_bounded((2. / 3.) * log_pos(nN));
   double aux = 1.5 * T / eF;
   return nN * aux / sqrt(1. + aux * aux);
 }
@@ -299,7 +299,7 @@ __device__ __forceinline__ Spec spec_from_m1(double n0, double J0, double chi, d
                  14364.6331099737) + 5750.1878570758) + 1120.71610972194) + 1.60356108438235e-8;
     double den = y * (y * (y * (y * (y * (y + 1.0) + 38840.0743174942) - 99.9009680656931) -
                  171.874844843596) + 75.7101579899442) + 83.0160130941424;
-    et = log(num / den);
+    et = log_pos(num / den);
   } else if (y <= 0.7) {
     double num = y * (y * (y * (y * (y * (41.3836568203438 * y + 32.5515666786612) -
                  157.774993512235) + 66.5726772426253) + 14.4883415579211) +
@@ -315,7 +315,7 @@ __device__ __forceinline__ Spec spec_from_m1(double n0, double J0, double chi, d
     double den = y * (y * (y * (y * (y * (y + 255.936658313629) + 9.42360945627147e-5) -
                  81.2467063138386) - 180.100197053091) - 89.0343496217014) +
                  143.849128123195;
-    et = exp(num / den);
+    et = exp_bounded(num / den);
   } else {
     et = 20.;
   }
@@ -331,7 +331,7 @@ __device__ __forceinline__ Spec spec_from_m1(double n0, double J0, double chi, d
   double Tnu = JJ / (nn_ * 3.6);
   d.invTf = 1. / Tnu;
   d.cf = 0.6;
-  d.bf = nn_ / (c::FourPiHc3 * gamma36 * exp(3.6 * log(Tnu)));
+  d.bf = nn_ / (c::FourPiHc3 * gamma36 * exp_bounded(3.6 * log_pos(Tnu)));
   d.lbf = 0.;
   return d;
 }
