@@ -309,10 +309,20 @@ def surrogate_eval(ctx, dpts, ref, fp64_ms, params, nquad, train_points, epochs,
     # as far as the FP64 path's own rounding noise goes)
     ctx.set_quadrature(nquad)
     noise = S.fp64_noise(ctx, dpts, params, ref).cpu().numpy()
-    caldata = S.calibration_data(ens, ctx, params, nquad, 1 << 21, seed=2000)
+    ncal = 1 << 22
+    caldata = S.calibration_data(ens, ctx, params, nquad, ncal, seed=2000)
+    calpts = W.make_points(ncal, seed=2000)
     import oracle as O
     for tol in tols:
-        cal = S.calibrate(ens, ctx, tol, params, nquad, coverage=coverage, log=log, data=caldata)
+        # the ensemble-spread rule on the first 2^21 calibration points, for
+        # comparison; then the judge (trained and calibrated on all 2^22),
+        # which the learned form runs with
+        half = tuple(x[..., :ncal // 2] for x in caldata)
+        spread_rule = S.calibrate(ens, ctx, tol, params, nquad, coverage=coverage, log=log,
+                                  data=half)
+        cal = S.calibrate_judge(ens, ctx, tol, params, nquad, data=caldata, pts=calpts,
+                                coverage=coverage, log=log)
+        cal["spread_rule_split_B"] = spread_rule["split_B"]
         bound = float(np.log1p(tol))
         ctx.set_quadrature(nquad)
         nfb = ens.m1_opacities(ctx, dpts, out, bound, W.FLAGS_ALL, params)  # warm-up
@@ -678,8 +688,9 @@ def main():
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral leg")
     ap.add_argument("--surrogate-train-points", type=int, default=10_000_000)
     ap.add_argument("--surrogate-epochs", type=int, default=48)
-    ap.add_argument("--surrogate-ln-loss", action="store_true",
-                    help="train the learned form on the squared error in natural-log units")
+    ap.add_argument("--surrogate-std-loss", action="store_true",
+                    help="train the learned form on the squared error in per-output "
+                         "standardised units instead of natural-log units")
     ap.add_argument("--spectral-layout", choices=("both", "struct", "fields"), default="both",
                     help="output layouts of the spectral leg (struct: SpectralOpacities records; "
                          "fields: [16][nnu][npts]); the first timed is the line's figure")
@@ -844,7 +855,7 @@ def main():
         line["surrogate"] = surrogate_eval(ctx, dpts, ref, b["kernel_ms"], params, args.nquad,
                                            args.surrogate_train_points, args.surrogate_epochs,
                                            tols=(1e-2, 3e-2, 1e-1),
-                                           ln_loss=args.surrogate_ln_loss)
+                                           ln_loss=not args.surrogate_std_loss)
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(pts, args.nquad, W.params_default_all(W.BREM_BRT06))

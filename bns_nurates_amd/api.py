@@ -215,6 +215,7 @@ def load_library():
         "bnsamd_mlp_predict": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
         "bnsamd_mlp_set_domain": ([vp, vp], C.c_int),
         "bnsamd_mlp_set_acceptance": ([vp, vp, vp], C.c_int),
+        "bnsamd_mlp_set_judge": ([vp, vp, C.c_double], C.c_int),
         "bnsamd_mlp_infer": ([vp, vp, i64, i64, vp, vp, vp, vp], C.c_int),
         "bnsamd_surrogate_m1_opacities": ([vp, vp, i64, i64, vp, C.c_float, C.POINTER(_Flags),
                                            C.POINTER(_Params), C.c_int, vp, C.POINTER(i64)],

@@ -283,7 +283,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       const double nb = vin[0], yp = vin[2], yn = vin[3];
       double rho = nb * c::Mb;
       double q = 2.0778e+02 * 0.5 * c::MeV * (yn * yn + yp * yp + (28. / 3.) * yn * yp) *
-                 (rho * rho) * exp_bounded(5.5 * log_pos(T));
+                 (rho * rho) * exp_any(5.5 * log_pos(T));
       double T2 = T * T, T7 = T2 * T2 * T2 * T;
       brt_pre = c::HC6FourPiSq * c::BremC4BRT06 * (q / T7);
     } else if (do_brem && A.params.brem_impl == 0) {

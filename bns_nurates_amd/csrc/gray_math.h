@@ -223,6 +223,17 @@ BNS_HD double log_pos(double x) {
 #endif
 }
 
+// e^x for any x: the bounded form for |x| < 746 (where it also returns the
+// library's underflow to 0 and overflow to inf), the library exponential for
+// larger, infinite or NaN arguments
+BNS_HD double exp_any(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return fabs(x) < 746. ? exp_bounded(x) : exp(x);
+#else
+  return std::exp(x);
+#endif
+}
+
 // FermiDistr (functions.hpp:4376-4395) given the already-formed argument
 // (e - mu) / T.  Written without a branch: 1/(e^a + 1) = e^-a/(e^-a + 1);
 // the denominator lies in [1, 2] and the numerator in [e^-700, 1].

@@ -419,6 +419,14 @@ struct InferArgs {
   const float* w[kMaxMembers];       // FP32 blobs (biases)
   int members;
   float threshold;                   // predicted log error that triggers fallback
+  // acceptance judge (bnsamd_mlp_set_judge): one network of the members'
+  // shape whose outputs are the predicted log margins of the 15 outputs;
+  // a point falls back when the largest exceeds jthreshold (jwp == nullptr:
+  // no judge)
+  const uint16_t* jwp;
+  const float* jw;
+  const bnsamd_mlp_norm* jnorm;
+  double jthreshold;
   double dU, dm_eff;                 // training-domain values of these point fields
   double mu_np_shift;                // dQ + dU of the beta rates in the training domain
   double beta_guard;                 // |mu_n - mu_p - mu_np_shift| < beta_guard T: fall back
@@ -507,6 +515,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   __shared__ __attribute__((aligned(16))) uint16_t X[kRows][kXLd];
   __shared__ __attribute__((aligned(16))) uint16_t H[kRows][kLds];
   __shared__ float spread[kRows][kNL];
+  __shared__ double jmargin[kRows][kNL];
   __shared__ float xs[kRows][kFeat];  // standardised features (linear output model)
   __shared__ int oob[kRows];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -564,10 +573,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // running ensemble sums of this lane's four outputs: outputs
   // 4 (lane / 16) + v of point 16 wave + lane % 16
   float sum[4] = {0.f, 0.f, 0.f, 0.f}, sq[4] = {0.f, 0.f, 0.f, 0.f};
+  float jy[4] = {0.f, 0.f, 0.f, 0.f};  // the judge's outputs (standardised)
   const int q4 = 4 * (lane >> 4);
-  for (int e = 0; e < a.members; ++e) {
-    const uint16_t* wp = a.wp[e];
-    const float* wf = a.w[e];
+  const int passes = a.members + (!CAL && a.jwp ? 1 : 0);
+  for (int e = 0; e < passes; ++e) {
+    const bool judge = e == a.members;
+    const uint16_t* wp = judge ? a.jwp : a.wp[e];
+    const float* wf = judge ? a.jw : a.w[e];
 #pragma unroll 1
     for (int l = 0; l < kLayers - 1; ++l) {
       floatx4 acc[4][4];
@@ -600,8 +612,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       float y = o[v] + bov[v];
-      sum[v] += y;
-      sq[v] += y * y;
+      if (judge) {
+        jy[v] = y;
+      } else {
+        sum[v] += y;
+        sq[v] += y * y;
+      }
     }
     __syncthreads();  // the next member overwrites H
   }
@@ -637,8 +653,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         a.cal_spread[c * a.ld + p] = zero ? -1.f : sd;
       }
     } else {
-      // predicted error of this log output (0 for a threshold zero)
-      spread[r][c] = (c < kOut && !zero) ? nm.err_a[c] + nm.err_k[c] * sd : 0.f;
+      // predicted error of this log output (0 for a threshold zero; a
+      // residual outside the training range rejects whatever err_k is)
+      spread[r][c] = (c < kOut && !zero)
+                         ? (sd < __builtin_huge_valf() ? nm.err_a[c] + nm.err_k[c] * sd : sd)
+                         : 0.f;
+      // the judge's predicted log margin, formed as the calibration form
+      // forms a one-member ensemble's output (a residual outside the
+      // judge's training range: unbounded); threshold zeros are exact
+      double jm = -__builtin_huge_val();
+      if (a.jwp && c < kOut && !zero) {
+        const bnsamd_mlp_norm& jn = *a.jnorm;
+        if (jy[v] >= jn.res_lo[c] && jy[v] <= jn.res_hi[c]) {
+          float lin = jn.lin[0][c];
+#pragma unroll
+          for (int i = 0; i < kFeat; ++i) lin = fmaf(jn.lin[1 + i][c], xs[r][i], lin);
+          jm = double(lin) + double(jy[v] * jn.out_std[c] + jn.out_mean[c]);
+        } else {
+          jm = __builtin_huge_val();
+        }
+      }
+      if (c < kNL) jmargin[r][c] = jm;
     }
   }
   if constexpr (CAL) {
@@ -650,7 +685,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     float s = 0.f;
 #pragma unroll
     for (int c = 0; c < kOut; ++c) s = fmaxf(s, spread[t][c]);
-    int fb = oob[t] || !(s <= a.threshold);
+    double jmax = -__builtin_huge_val();
+    if (a.jwp) {
+#pragma unroll
+      for (int c = 0; c < kOut; ++c) jmax = fmax(jmax, jmargin[t][c]);
+    }
+    int fb = oob[t] || !(s <= a.threshold) || (a.jwp && !(jmax <= a.jthreshold));
     oob[t] = fb;
     int64_t p = p0 + t;
     if (fb && p < a.npts) {
@@ -753,6 +793,13 @@ struct bnsamd_mlp {
   double* fb_pts = nullptr;
   double* fb_out = nullptr;
   bool packed_valid = false;
+  // acceptance judge: a copy of another instance's member 0 (FP32 blob,
+  // packed BF16 weights) and of its normalisation block
+  bool judge_set = false;
+  float* jw = nullptr;
+  uint16_t* jwp = nullptr;
+  float* jnorm = nullptr;
+  double jthreshold = 0.;
 };
 
 namespace {
@@ -999,6 +1046,9 @@ void bnsamd_mlp_destroy(bnsamd_mlp* m) {
   fr(m->fb_count);
   fr(m->fb_pts);
   fr(m->fb_out);
+  fr(m->jw);
+  fr(m->jwp);
+  fr(m->jnorm);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -1235,6 +1285,54 @@ int bnsamd_mlp_set_acceptance(bnsamd_mlp* m, const float* err_a, const float* er
   return BNSAMD_OK;
 }
 
+int bnsamd_mlp_set_judge(bnsamd_mlp* m, bnsamd_mlp* judge, double threshold) {
+  if (!m || judge == m) return BNSAMD_ERR_INVALID;
+  (void)hipSetDevice(m->device);
+  // an inference still queued may read the judge block
+  if (hipDeviceSynchronize() != hipSuccess)
+    return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_set_judge: synchronize");
+  if (!judge) {
+    m->judge_set = false;
+    return BNSAMD_OK;
+  }
+  if (judge->device != m->device)
+    return mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_mlp_set_judge: judge on another device");
+  if (std::isnan(threshold))
+    return mlp_fail(m, BNSAMD_ERR_INVALID, "bnsamd_mlp_set_judge: NaN threshold");
+  // the judge reads the features the ensemble standardises: same feature
+  // statistics and training box
+  const bnsamd_mlp_norm &a = m->hnorm, &b = judge->hnorm;
+  if (std::memcmp(a.feat_mean, b.feat_mean, sizeof(a.feat_mean)) != 0 ||
+      std::memcmp(a.feat_std, b.feat_std, sizeof(a.feat_std)) != 0 ||
+      std::memcmp(a.box_lo, b.box_lo, sizeof(a.box_lo)) != 0 ||
+      std::memcmp(a.box_hi, b.box_hi, sizeof(a.box_hi)) != 0)
+    return mlp_fail(m, BNSAMD_ERR_INVALID,
+                    "bnsamd_mlp_set_judge: feature standardisation differs from the ensemble's");
+  if (!m->jw) {
+    if (hipMalloc(&m->jw, sizeof(float) * kParams) != hipSuccess ||
+        hipMalloc(&m->jwp, sizeof(uint16_t) * kPacked) != hipSuccess ||
+        hipMalloc(&m->jnorm, sizeof(bnsamd_mlp_norm)) != hipSuccess)
+      return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_set_judge: hipMalloc");
+  }
+  // the judge's training has finished; copy its member 0 and pack it
+  hipError_t e = sync_all(judge);
+  if (e == hipSuccess)
+    e = hipMemcpy(m->jw, judge->w[0], sizeof(float) * kParams, hipMemcpyDeviceToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(m->jnorm, &judge->hnorm, sizeof(bnsamd_mlp_norm), hipMemcpyHostToDevice);
+  for (int l = 0; l < kLayers && e == hipSuccess; ++l) {
+    int64_t n = int64_t(kLayerK[l]) * kLayerN[l];
+    hipLaunchKernelGGL(k_pack, dim3(unsigned((n + 255) / 256)), dim3(256), 0, nullptr, kLayerK[l],
+                       kLayerN[l], m->jw + kWOff[l], m->jwp + kPOff[l], l == 0 ? 1 : 0);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return mlp_fail(m, BNSAMD_ERR_HIP, "bnsamd_mlp_set_judge: copy");
+  m->jthreshold = threshold;
+  m->judge_set = true;
+  return BNSAMD_OK;
+}
+
 namespace {
 // inference arguments shared by the learned form and the raw predictions;
 // packs the weights when training changed them
@@ -1261,6 +1359,12 @@ int infer_setup(bnsamd_mlp* m, hipStream_t s, int64_t npts, int64_t ld, const do
     a.w[e] = m->w[e];
   }
   a.members = m->members;
+  if (m->judge_set) {
+    a.jwp = m->jwp;
+    a.jw = m->jw;
+    a.jnorm = reinterpret_cast<const bnsamd_mlp_norm*>(m->jnorm);
+    a.jthreshold = m->jthreshold;
+  }
   a.dU = m->domain.dU;
   a.dm_eff = m->domain.dm_eff;
   const bnsamd_params& pr = m->domain.params;

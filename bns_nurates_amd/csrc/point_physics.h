@@ -253,7 +253,7 @@ __device__ __forceinline__ void beta_em_ab(double nu, int s, const BetaPoint& bp
 __device__ __forceinline__ double eta_nn_sc(double nb, double T, double yN) {
   double nN = yN * nb;
   if (nN <= 0.) return 0.;
-  double eF = c::IsoEF * exp_bounded((2. / 3.) * log_pos(nN));
+  double eF = c::IsoEF * exp_any((2. / 3.) * log_pos(nN));
   double aux = 1.5 * T / eF;
   return nN * aux / sqrt(1. + aux * aux);
 }
@@ -315,7 +315,7 @@ __device__ __forceinline__ Spec spec_from_m1(double n0, double J0, double chi, d
     double den = y * (y * (y * (y * (y * (y + 255.936658313629) + 9.42360945627147e-5) -
                  81.2467063138386) - 180.100197053091) - 89.0343496217014) +
                  143.849128123195;
-    et = exp_bounded(num / den);
+    et = exp_any(num / den);
   } else {
     et = 20.;
   }
@@ -331,7 +331,7 @@ __device__ __forceinline__ Spec spec_from_m1(double n0, double J0, double chi, d
   double Tnu = JJ / (nn_ * 3.6);
   d.invTf = 1. / Tnu;
   d.cf = 0.6;
-  d.bf = nn_ / (c::FourPiHc3 * gamma36 * exp_bounded(3.6 * log_pos(Tnu)));
+  d.bf = nn_ / (c::FourPiHc3 * gamma36 * exp_any(3.6 * log_pos(Tnu)));
   d.lbf = 0.;
   return d;
 }

@@ -206,7 +206,7 @@ k_spectral_f64(SpectralKernelArgs A) {
     const double nb = vin[0], yp = vin[2], yn = vin[3];
     double rho = nb * c::Mb;
     double q = 2.0778e+02 * 0.5 * c::MeV * (yn * yn + yp * yp + (28. / 3.) * yn * yp) *
-               (rho * rho) * exp_bounded(5.5 * log_pos(T));
+               (rho * rho) * exp_any(5.5 * log_pos(T));
     double T2 = T * T, T7 = T2 * T2 * T2 * T;
     brt_pre = c::HC6FourPiSq * c::BremC4BRT06 * (q / T7);
   } else if (do_brem && A.params.brem_impl == 0) {

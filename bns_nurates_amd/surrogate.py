@@ -140,6 +140,9 @@ class Ensemble:
         self.norm = None
 
     def close(self):
+        if getattr(self, "judge", None) is not None:
+            self.judge.close()
+            self.judge = None
         if getattr(self, "h", None):
             self.lib.bnsamd_mlp_destroy(self.h)
             self.h = None
@@ -232,6 +235,14 @@ class Ensemble:
         self._check(self.lib.bnsamd_mlp_set_acceptance(self.h, a.ctypes.data, k.ctypes.data),
                     "bnsamd_mlp_set_acceptance")
         self.err_a, self.err_k = a.copy(), k.copy()
+
+    def set_judge(self, judge: "Ensemble | None", threshold: float = 0.0):
+        """Acceptance judge (bnsamd_mlp_set_judge): member 0 of `judge`
+        (copied), rejecting a point whose largest predicted log margin
+        exceeds threshold; None removes it."""
+        self._check(self.lib.bnsamd_mlp_set_judge(self.h, judge.h if judge is not None else None,
+                                                  float(threshold)), "bnsamd_mlp_set_judge")
+        self.judge_threshold = None if judge is None else float(threshold)
 
     def infer(self, ctx, pts):
         """Raw predictions on a device point block pts [21, >= n]: mean log
@@ -530,6 +541,7 @@ def calibrate(ens: "Ensemble", ctx, tol: float, params, nquad: int, ncal: int = 
     a_set = np.where(np.isfinite(a_set) & (a_set < 3e38), a_set, 3e38).astype(np.float32)
     k_set = np.where(np.isfinite(k_set) & (k_set < 3e38), k_set, 3e38).astype(np.float32)
     ens.set_acceptance(a_set, k_set)
+    ens.set_judge(None)  # this rule alone decides
     rep = {"tol": tol, "error_bound_log": b, "rule": name,
            "candidates_split_A_accepted": {k: v[1] for k, v in fits.items()},
            "coverage_target": coverage, "calibration_points": n,
@@ -548,6 +560,138 @@ def calibrate(ens: "Ensemble", ctx, tol: float, params, nquad: int, ncal: int = 
                     "violation_rate_ucb95": float(f"{clopper_pearson_upper(nv, na):.3g}")}
     if log:
         log(f"calibrate tol={tol}: rule {name}, {rep['split_B']}")
+    return rep
+
+
+# ---- acceptance judge -------------------------------------------------------------
+# The ensemble's spread separates good from bad predictions poorly: at tol 0.1
+# the network has every output within the bar on ~97% of the points inside its
+# domain, but no spread threshold isolates the other 3%.  A judge network of
+# the members' shape (15 features -> 15 outputs), trained by the library's own
+# kernels on held-out points, predicts each output's log margin to the bar;
+# a point is accepted when every predicted margin lies below a threshold
+# calibrated on further held-out points.
+MARGIN_LO, MARGIN_HI = -8.0, 4.0
+
+
+def judge_margins(mean: np.ndarray, spread: np.ndarray, ref: np.ndarray, noise: np.ndarray,
+                  tol: float) -> np.ndarray:
+    """Log margins [15, n] of the ensemble's predictions exp(mean) to the bar
+    |pred - ref| <= tol |ref| + NOISE_K D, clamped to [MARGIN_LO, MARGIN_HI]:
+    log(|pred - ref| / (tol |ref| + NOISE_K D)); exact outputs (threshold
+    zeros, spread < 0) at MARGIN_LO."""
+    r, D = ref[OUT_ROWS], noise[OUT_ROWS]
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        p = np.where(spread < 0, 0.0, np.exp(mean))
+        m = np.log(np.abs(p - r) / (tol * np.abs(r) + NOISE_K * D))
+    m = np.where(np.isnan(m), MARGIN_LO, m)  # 0 / 0: exact
+    return np.clip(m, MARGIN_LO, MARGIN_HI)
+
+
+def _norm_arrays(ens: "Ensemble"):
+    n = ens.norm
+    return (np.array(n.feat_mean[:], np.float64), np.array(n.feat_std[:], np.float64),
+            np.array(n.box_lo[:], np.float64), np.array(n.box_hi[:], np.float64))
+
+
+def train_judge(ens: "Ensemble", ctx, tol: float, pts: np.ndarray, data, rows, epochs: int = 16,
+                batch: int = 4096, lr0: float = 4e-3, lr1: float = 2e-5, seed: int = 3000,
+                log=None) -> "Ensemble":
+    """A one-member network that predicts judge_margins() of ens at tol from
+    the 15 features, trained (library kernels, MSE on the standardised
+    residual of a linear model, as the members) on the points `rows` of the
+    calibration points pts / data (calibration_data()) that lie inside the
+    training box and domain.  Its feature standardisation and box are the
+    ensemble's, as bnsamd_mlp_set_judge requires."""
+    import torch
+    mean, spread, outside, ref, noise = data
+    rows = np.asarray(rows)
+    ok = rows[outside[rows] == 0]
+    t = judge_margins(mean[:, ok], spread[:, ok], ref[:, ok], noise[:, ok], tol).T
+    with np.errstate(divide="ignore"):
+        f = features(pts[:, ok])
+    fm, fs, lo, hi = _norm_arrays(ens)
+    xn = (f - fm.astype(np.float32)) / fs.astype(np.float32)
+    X1 = np.c_[np.ones(len(xn)), xn]
+    A = masked_linear_fit(X1, t)
+    res = t - X1 @ A
+    del X1
+    tm, ts = res.mean(0), res.std(0)
+    ts = np.where(ts > 0, ts, 1.0)
+    ys = (res - tm) / ts
+    j = Ensemble(ctx.device, members=1, max_batch=batch)
+    j.set_norm(fm, fs, tm, ts, lo, hi, A, ys.min(0), ys.max(0))
+    d = ens.domain
+    j.domain = d
+    j._check(j.lib.bnsamd_mlp_set_domain(j.h, C.byref(d)), "bnsamd_mlp_set_domain")
+    j.set_params(0, init_params(seed))
+    perm = np.random.default_rng(seed).permutation(len(xn)).astype(np.int32)[None, :]
+    hist = j.train_all(torch.from_numpy(xn.astype(np.float32)).cuda(),
+                       torch.from_numpy(ys.astype(np.float32)).cuda(),
+                       torch.from_numpy(np.ascontiguousarray(perm)).cuda(), epochs, batch, lr0, lr1)
+    j.n_train, j.history = int(len(xn)), [hist[0]]
+    if log:
+        log(f"judge tol={tol}: {len(xn)} points, epoch losses {np.array2string(hist[0], precision=4)}")
+    return j
+
+
+def judge_scores(j: "Ensemble", ctx, pts: np.ndarray, outside: np.ndarray) -> np.ndarray:
+    """Largest predicted margin of each point (inf outside the training box
+    or domain, or where a predicted residual leaves the judge's training
+    range; exact outputs excepted), as the learned form's kernel forms it."""
+    import torch
+    d = torch.from_numpy(np.ascontiguousarray(pts)).cuda()
+    m, sp, out = (x.cpu().numpy() for x in j.infer(ctx, d))
+    del d
+    sc = np.where(sp < 0, -np.inf, np.where(np.isfinite(sp), m, np.inf)).max(axis=0)
+    return np.where((outside == 0) & (out == 0), sc, np.inf)
+
+
+def calibrate_judge(ens: "Ensemble", ctx, tol: float, params, nquad: int, data=None,
+                    pts=None, ncal: int = 1 << 22, seed: int = 2000, coverage: float = 0.999,
+                    minacc: int = 3000, epochs: int = 16, log=None) -> dict:
+    """Trains a judge for tol on the first half of ncal held-out points,
+    calibrates its threshold on the third quarter (the largest that accepts
+    at most (1 - coverage) / 2 points with an output beyond the bar, and at
+    least minacc points), reports on the last quarter, and sets it on the
+    ensemble (the spread rule off: err_a = err_k = 0).  data / pts:
+    calibration_data() and the points it was made on (seed `seed`)."""
+    if data is None:
+        data = calibration_data(ens, ctx, params, nquad, ncal, seed)
+        pts = W.make_points(ncal, seed=seed)
+    mean, spread, outside, ref, noise = data
+    n = ref.shape[1]
+    A1, A2, B = np.arange(0, n // 2), np.arange(n // 2, 3 * n // 4), np.arange(3 * n // 4, n)
+    j = train_judge(ens, ctx, tol, pts, data, A1, epochs=epochs, log=log)
+    ctx.set_quadrature(nquad)
+    _, bad, _, _ = prediction_errors(mean, spread, outside, ref, noise, tol)
+    anybad = np.any(bad, axis=0)
+    target = (1.0 - coverage) / 2
+    sA2 = judge_scores(j, ctx, pts[:, A2], outside[A2])
+    t = _accept_threshold(sA2, anybad[A2], target, minacc)
+    rep = {"tol": tol, "rule": "judge", "coverage_target": coverage, "calibration_points": n,
+           "judge_train_points": j.n_train, "judge_epochs": epochs,
+           "judge_final_loss": float(f"{j.history[0][-1]:.4g}"),
+           "bar": f"|pred - ref| <= tol |ref| + {NOISE_K:g} D (D: FP64 change under "
+                  f"2^-46 perturbations of T and mu_e)",
+           "threshold": float(t) if np.isfinite(t) else None,
+           "inside_frac": round(float((outside == 0).mean()), 4)}
+    for tag, cols, sc in (("split_A2", A2, sA2),
+                          ("split_B", B, judge_scores(j, ctx, pts[:, B], outside[B]))):
+        acc = sc <= t
+        na, nv = int(acc.sum()), int((acc & anybad[cols]).sum())
+        rep[tag] = {"accepted_frac": round(na / len(cols), 5), "accepted": na, "violations": nv,
+                    "violation_rate_ucb95": float(f"{clopper_pearson_upper(nv, na):.3g}")}
+    # the spread rule off; a threshold of -inf rejects every point.  The
+    # library keeps its own copy of the judge; ens.judge keeps this one for
+    # diagnostics (judge_scores) until the next calibration or close()
+    ens.set_acceptance(np.zeros(OUT), np.zeros(OUT))
+    ens.set_judge(j, t if np.isfinite(t) else -np.inf)
+    if getattr(ens, "judge", None) is not None:
+        ens.judge.close()
+    ens.judge = j
+    if log:
+        log(f"calibrate_judge tol={tol}: threshold {rep['threshold']}, {rep['split_B']}")
     return rep
 
 

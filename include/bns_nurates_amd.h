@@ -378,6 +378,21 @@ int bnsamd_mlp_set_domain(bnsamd_mlp* m, const bnsamd_mlp_domain* domain);
  * (bns_nurates_amd/surrogate.py calibrate). */
 int bnsamd_mlp_set_acceptance(bnsamd_mlp* m, const float* err_a, const float* err_k);
 
+/* Acceptance judge of the learned form: a network of the members' shape,
+ * member 0 of `judge`, whose outputs (through judge's own output model,
+ * bnsamd_mlp_norm) are the predicted log margins
+ * log(|pred_o - ref_o| / (tol |ref_o| + 16 D_o)) of the ensemble's 15
+ * outputs at one tolerance (bns_nurates_amd/surrogate.py train_judge).
+ * When set, a point is also recomputed by the FP64 path when its largest
+ * predicted margin exceeds threshold (outputs the FP64 path sets to zero
+ * below its density thresholds excepted; a residual outside the judge's
+ * training range counts as +inf).  The judge's parameters and normalisation
+ * block are copied: `judge` may be changed or destroyed afterwards.  Its
+ * feature standardisation and training box must equal the ensemble's
+ * (BNSAMD_ERR_INVALID otherwise).  judge == NULL removes the judge.  No
+ * counterpart in the reference. */
+int bnsamd_mlp_set_judge(bnsamd_mlp* m, bnsamd_mlp* judge, double threshold);
+
 /* The ensemble's raw predictions on npts points (calibration and
  * diagnostics): d_logmean[15][ld] the mean log output (nue, anue, nux x
  * eta_0, eta, kappa_0_a, kappa_a, kappa_s), d_spread[15][ld] its spread s_o
@@ -392,9 +407,10 @@ int bnsamd_mlp_infer(bnsamd_context* ctx, bnsamd_mlp* m, int64_t npts, int64_t l
  * the context's stream): every member of the ensemble evaluates each point.
  * A point is recomputed by the FP64 path when its features leave the
  * training box, it leaves the training domain (bnsamd_mlp_domain), a
- * predicted residual leaves the range seen in training, or the predicted
+ * predicted residual leaves the range seen in training, the predicted
  * error of any output (bnsamd_mlp_set_acceptance) exceeds error_bound
- * (natural-log units: log1p(tol) for a relative tolerance tol); the other
+ * (natural-log units: log1p(tol) for a relative tolerance tol), or a judge
+ * is set and rejects it (bnsamd_mlp_set_judge); the other
  * points take the ensemble mean.  When flags, params, stim_abs or the
  * context's quadrature orders differ from the training domain, every point
  * is recomputed.  Fails with BNSAMD_ERR_INVALID before

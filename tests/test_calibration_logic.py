@@ -13,6 +13,9 @@ class _Ens:
     def set_acceptance(self, a, k):
         self.a, self.k = np.asarray(a, np.float64), np.asarray(k, np.float64)
 
+    def set_judge(self, judge, threshold=0.0):
+        self.judge = judge
+
 
 def _data(n, seed, wrong=2e-4):
     rng = np.random.default_rng(seed)
@@ -66,3 +69,33 @@ def test_accept_threshold_ties_and_minimum():
     assert S._accept_threshold(score, bad, 0.3, 1) == 3.0
     assert S._accept_threshold(score, bad, 0.3, 5) == -np.inf
     assert S._accept_threshold(score, bad, 0.0, 1) == -np.inf
+
+
+def test_judge_margins():
+    """The judge's targets: log(|pred - ref| / (tol |ref| + NOISE_K D)),
+    clamped; outputs the FP64 path sets to zero (spread < 0) and exact
+    predictions at the lower clamp; a wrong value where the bar is zero at
+    the upper one."""
+    tol = 0.1
+    n = 6
+    ref = np.zeros((20, n))
+    noise = np.zeros((20, n))
+    rows = S.OUT_ROWS
+    ref[rows] = 2.0
+    mean = np.full((S.OUT, n), np.log(2.0))
+    spread = np.zeros((S.OUT, n), np.float32)
+    mean[0, 1] = np.log(2.0 * 1.05)       # 5% off at tol 10%: margin log(0.1 / 0.2)
+    mean[0, 2] = np.log(2.0 * 3.0)        # 200% off: log(4 / 0.2) = log 20, below the clamp
+    noise[rows[0], 3] = 0.5               # the bar grows with the FP64 noise
+    mean[0, 3] = np.log(2.0 * 1.5)        # |d| = 1, bar 0.2 + 16 * 0.5 = 8.2
+    spread[1, 4] = -1.0                   # a threshold zero: exact
+    ref[rows[1], 4] = 0.0
+    ref[rows[2], 5] = 0.0                 # bar 0 and a positive prediction
+    m = S.judge_margins(mean, spread, ref, noise, tol)
+    assert m.shape == (S.OUT, n)
+    assert m[0, 0] == S.MARGIN_LO          # exact
+    assert np.isclose(m[0, 1], np.log(0.1 / 0.2))
+    assert np.isclose(m[0, 2], min(np.log(4.0 / 0.2), S.MARGIN_HI))
+    assert np.isclose(m[0, 3], np.log(1.0 / 8.2))
+    assert m[1, 4] == S.MARGIN_LO
+    assert m[2, 5] == S.MARGIN_HI

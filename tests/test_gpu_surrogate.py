@@ -261,3 +261,85 @@ def test_out_of_box_points_fall_back(trained):
     nfb = ens.m1_opacities(ctx, d, out, 1.0, W.FLAGS_ALL, prm)
     assert nfb >= 64
     assert np.array_equal(out.cpu().numpy()[:, :64], ref[:, :64])
+
+
+@pytest.fixture(scope="module")
+def judge_data(trained):
+    ctx, ens, prm = trained
+    n = 1 << 21
+    return S.calibration_data(ens, ctx, prm, NQ, n, seed=901), W.make_points(n, seed=901)
+
+
+@pytest.mark.parametrize("tol", [3e-1, 1e-1])
+def test_judge_rule_holds_the_bar(trained, judge_data, tol):
+    """The acceptance judge (a one-member network of the members' shape that
+    predicts each output's log margin to the bar, trained by the library's
+    kernels on held-out points, its threshold calibrated on further ones):
+    on 2^20 new points the learned form accepts exactly the points whose
+    largest predicted margin (calibration form of the judge) is within the
+    threshold and lie in the training domain; accepted outputs meet the bar
+    at the stated coverage; fallback points carry the FP64 bits; threshold
+    zeros stay exact."""
+    ctx, ens, prm = trained
+    data, cpts = judge_data
+    rep = S.calibrate_judge(ens, ctx, tol, prm, NQ, data=data, pts=cpts, coverage=COVERAGE,
+                            minacc=1000, epochs=8, log=say)
+    try:
+        pts = W.make_points(1 << 20, seed=78)
+        d = torch.from_numpy(pts).cuda()
+        ctx.set_quadrature(NQ)
+        ref = ctx.compute_m1_opacities(d, flags=W.FLAGS_ALL, params=prm)
+        noise = S.fp64_noise(ctx, d, prm, ref).cpu().numpy()
+        out = torch.full((20, pts.shape[1]), float("nan"), dtype=torch.float64, device="cuda")
+        nfb = ens.m1_opacities(ctx, d, out, float(np.log1p(tol)), W.FLAGS_ALL, prm)
+        o, r = out.cpu().numpy(), ref.cpu().numpy()
+        acc = ~np.all(o == r, axis=0)
+        _, _, outside = (x.cpu().numpy() for x in ens.infer(ctx, d))
+        t = rep["threshold"] if rep["threshold"] is not None else -np.inf
+        score = S.judge_scores(ens.judge, ctx, pts, outside)
+        bad = np.any(S.violations(o, r, noise, tol), axis=0)
+        nacc, nviol = int(acc.sum()), int((acc & bad).sum())
+        say(f"judge tol {tol}: {rep['split_B']}; held out: {nacc} accepted, {nfb} fell back, "
+            f"{nviol} beyond the bar")
+        assert np.isfinite(o).all()
+        assert nfb == pts.shape[1] - nacc
+        assert np.array_equal(acc, score <= t)
+        assert _consistent_with_rate(nviol, nacc, 1.0 - COVERAGE), (nviol, nacc)
+        if tol >= 3e-1:
+            assert nacc > 0
+        zero = r == 0
+        assert np.array_equal(o[zero], r[zero])
+    finally:
+        ens.set_judge(None)
+
+
+def test_judge_checks_and_rejects(trained, judge_data):
+    """bnsamd_mlp_set_judge refuses a judge whose feature standardisation
+    differs from the ensemble's; a judge threshold of -inf sends every point
+    to the FP64 path (its bits exactly)."""
+    from bns_nurates_amd import api
+    ctx, ens, prm = trained
+    data, cpts = judge_data
+    S.calibrate_judge(ens, ctx, 0.3, prm, NQ, data=data, pts=cpts, coverage=COVERAGE,
+                      minacc=1000, epochs=2)
+    try:
+        other = S.Ensemble(ctx.device, members=1)
+        n = ens.norm
+        fm = np.array(n.feat_mean[:], np.float64)
+        other.set_norm(fm + 1.0, np.array(n.feat_std[:]), np.zeros(S.OUT), np.ones(S.OUT),
+                       np.array(n.box_lo[:]), np.array(n.box_hi[:]),
+                       np.zeros((S.FEAT + 1, S.OUT)), -np.ones(S.OUT), np.ones(S.OUT))
+        with pytest.raises(api.BnsAmdError):
+            ens.set_judge(other, 0.0)
+        other.close()
+        ens.set_judge(ens.judge, -np.inf)
+        pts = W.make_points(4096, seed=79)
+        d = torch.from_numpy(pts).cuda()
+        ctx.set_quadrature(NQ)
+        ref = ctx.compute_m1_opacities(d, flags=W.FLAGS_ALL, params=prm).cpu().numpy()
+        out = torch.empty((20, pts.shape[1]), dtype=torch.float64, device="cuda")
+        nfb = ens.m1_opacities(ctx, d, out, float(np.log1p(0.3)), W.FLAGS_ALL, prm)
+        assert nfb == pts.shape[1]
+        assert np.array_equal(out.cpu().numpy(), ref)
+    finally:
+        ens.set_judge(None)
