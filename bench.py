@@ -687,7 +687,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned form")
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral leg")
     ap.add_argument("--surrogate-train-points", type=int, default=10_000_000)
-    ap.add_argument("--surrogate-epochs", type=int, default=48)
+    ap.add_argument("--surrogate-epochs", type=int, default=96)
     ap.add_argument("--surrogate-std-loss", action="store_true",
                     help="train the learned form on the squared error in per-output "
                          "standardised units instead of natural-log units")
