@@ -24,8 +24,9 @@
 namespace bns {
 
 // slot of eta = mu_e / T in the per-point LDS block (after the 21 inputs and
-// 28 distribution values, before log T and 1 / T)
+// 28 distribution values, before log T and 1 / T), and of e^-eta (eta >= 0)
 constexpr int kEtaSlot = 61;
+constexpr int kEmEtaSlot = 60;
 
 // ---------------------------------------------------------------------------
 // The body of both kernel variants: MIX = false is the FP64 path, MIX = true
@@ -88,6 +89,10 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   if (lane == 63) s_in[kInvT] = 1. / T;
   if (lane == 62) s_in[kLogT] = log_pos(T);
   if (lane == 61) s_in[kEtaSlot] = mu_e / T;
+  if (lane == 60) {
+    const double e = mu_e / T;
+    s_in[kEmEtaSlot] = e >= 0. ? exp_any(-e) : 0.;
+  }
 
   // ---- per-point constants --------------------------------------------------
   const double eta = mu_e / T;
@@ -540,7 +545,19 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           // the deep tail of the exponential form, so the whole wavefront
           // skips the rationals (the same bits); elsewhere the wave-uniform
           // test costs more than it saves
-          if (lep == 1) {
+          if (lep == 1 && el <= 0.) {
+            // every argument -eta - v (v = w', w, |w - w'|) is <= 0: its
+            // exponential is e^-eta e^-v, e^-v from the cell table at the
+            // table's v and corrected to first order for this point's
+            // rounding of v (|v - v_tab| is a few ulps of v): three
+            // operations instead of an exponential per bundle
+            const double em = vin[kEmEtaSlot];
+            const double wt = hr[NP_W], wpt = hr[NP_WP];
+            const double ew = hr[NP_EW], ewp = hr[NP_EWP], ead = hr[NP_EAD];
+            fdi_bundle_ex<1, 5, true>(el - wp, em * fma(wpt - wp, ewp, ewp), A5);
+            fdi_bundle_ex<1, 5, true>(el - w, em * fma(wt - w, ew, ew), B5);
+            fdi_bundle_ex<3, 5, true>(el - ad, em * fma(fabs(wt - wpt) - ad, ead, ead), C3);
+          } else if (lep == 1) {
             fdi_bundle<1, 5, true>(el - wp, A5);
             fdi_bundle<1, 5, true>(el - w, B5);
             fdi_bundle<3, 5, true>(el - ad, C3);

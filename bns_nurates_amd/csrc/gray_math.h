@@ -542,6 +542,34 @@ BNS_HD void fdi_bundle(double y, double* F) {
   if (y > 0.) FdiReflect<KLO>::template apply<KHI>(y, F);
 }
 
+// The bundle at an argument y <= 0 whose exponential e^y the caller supplies
+// (formed otherwise than by exp_bounded, within a few ulps of it): the same
+// forms, branch decisions and operations as fdi_bundle after its
+// exponential; no reflection (y <= 0).
+template <int KLO, int KHI = 5, bool DEEP = false>
+BNS_HD void fdi_bundle_ex(double y, double ex_in, double* F) {
+  const double x = y;
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (DEEP && __all(x < kFdiDeep)) {
+    FdiDeepUnroll<KLO>::template eval<KHI>(ex_in, F);
+    return;
+  }
+#endif
+  bool ebr = x < -2.;
+  bool rbr = !ebr && (x <= 0.);  // NaN arguments give 0, as the reference
+  double ex = 0., A = rbr ? 1. : 0., t, s;
+  if (ebr) {
+    ex = ex_in;
+    A = -ex;
+    t = ex * c::E2;
+    s = 0.;
+  } else {
+    s = rbr ? -0.5 * x : 0.;
+    t = 1. - s;
+  }
+  FdiUnroll<KLO>::template eval<KHI>(ebr, ex, A, t, s, F);
+}
+
 template <int K>
 BNS_HD double fdi(double y) {
   double F[6];

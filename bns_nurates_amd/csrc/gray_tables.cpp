@@ -113,6 +113,9 @@ TableLayout build_gray_tables(int n1, const double* x1, const double* w1, int n2
         r[NP_HALF] = half;
         r[NP_LW] = std::log(w);
         r[NP_LWP] = std::log(wp);
+        r[NP_EW] = std::exp(-w);
+        r[NP_EWP] = std::exp(-wp);
+        r[NP_EAD] = std::exp(-std::fabs(w - wp));
       }
     }
   }

@@ -56,7 +56,10 @@ enum {
   NP_HALF,    // 0: nu = t x_i (1 -+ x_j)/2, 1: nu = t (1 -+ x_j)/(2 x_i)
   NP_LW,      // log(w): log(nu) = log(T) + log(w)
   NP_LWP,     // log(w')
-  NP_REC = 14
+  NP_EW,      // e^-w, e^-w' and e^-|w - w'|: with e^-|eta| of the point the
+  NP_EWP,     // exponentials of the Fermi integrals at -|eta| - w, -|eta| - w'
+  NP_EAD,     // and -|eta| - |w - w'| (see the NEPS loop)
+  NP_REC = 17
 };
 
 struct TableLayout {
