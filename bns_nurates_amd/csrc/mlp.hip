@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -1309,11 +1310,24 @@ int bnsamd_mlp_set_judge(bnsamd_mlp* m, bnsamd_mlp* judge, double threshold) {
     return mlp_fail(m, BNSAMD_ERR_INVALID,
                     "bnsamd_mlp_set_judge: feature standardisation differs from the ensemble's");
   if (!m->jw) {
+    // first judge of this ensemble: on failure nothing is kept (no judge)
     if (hipMalloc(&m->jw, sizeof(float) * kParams) != hipSuccess ||
         hipMalloc(&m->jwp, sizeof(uint16_t) * kPacked) != hipSuccess ||
-        hipMalloc(&m->jnorm, sizeof(bnsamd_mlp_norm)) != hipSuccess)
+        hipMalloc(&m->jnorm, sizeof(bnsamd_mlp_norm)) != hipSuccess) {
+      (void)hipFree(m->jw);
+      (void)hipFree(m->jwp);
+      (void)hipFree(m->jnorm);
+      m->jw = nullptr;
+      m->jwp = nullptr;
+      m->jnorm = nullptr;
+      m->judge_set = false;
       return mlp_fail(m, BNSAMD_ERR_NOMEM, "bnsamd_mlp_set_judge: hipMalloc");
+    }
   }
+  // while the blocks are rewritten the judge rejects every point: a copy
+  // that fails leaves that judge, never a partly copied one
+  m->jthreshold = -std::numeric_limits<double>::infinity();
+  m->judge_set = true;
   // the judge's training has finished; copy its member 0 and pack it
   hipError_t e = sync_all(judge);
   if (e == hipSuccess)

@@ -389,8 +389,10 @@ int bnsamd_mlp_set_acceptance(bnsamd_mlp* m, const float* err_a, const float* er
  * training range counts as +inf).  The judge's parameters and normalisation
  * block are copied: `judge` may be changed or destroyed afterwards.  Its
  * feature standardisation and training box must equal the ensemble's
- * (BNSAMD_ERR_INVALID otherwise).  judge == NULL removes the judge.  No
- * counterpart in the reference. */
+ * (BNSAMD_ERR_INVALID otherwise).  judge == NULL removes the judge.  A call
+ * rejected by these checks changes nothing; one that fails while copying
+ * leaves a judge that sends every point to the FP64 path.  No counterpart
+ * in the reference. */
 int bnsamd_mlp_set_judge(bnsamd_mlp* m, bnsamd_mlp* judge, double threshold);
 
 /* The ensemble's raw predictions on npts points (calibration and
