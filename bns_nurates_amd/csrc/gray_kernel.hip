@@ -27,6 +27,7 @@ namespace bns {
 // 28 distribution values, before log T and 1 / T), and of e^-eta (eta >= 0)
 constexpr int kEtaSlot = 61;
 constexpr int kEmEtaSlot = 60;
+constexpr int kNpOffSlot = 59;
 
 // ---------------------------------------------------------------------------
 // The body of both kernel variants: MIX = false is the FP64 path, MIX = true
@@ -58,6 +59,11 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   __shared__ double s_brem[6];
   __shared__ double s_Fe[2][6], s_fd0[2];
   __shared__ HrChan s_hc[3];
+  // the n = 16 instance holds its point's order of the NEPS half-cells here
+  // (1 KB, within the 10 KB of four workgroups per SIMD); others read it
+  // from the table block
+  constexpr bool kPermLds = NQ == 16;
+  __shared__ uint4 s_perm[kPermLds ? 2 * NQ * NQ * 2 / 16 : 1];
 
   const int lane = threadIdx.x;
   const int64_t p = xcd_point(blockIdx.x, gridDim.x);
@@ -92,6 +98,14 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   if (lane == 60) {
     const double e = mu_e / T;
     s_in[kEmEtaSlot] = e >= 0. ? exp_any(-e) : 0.;
+  }
+  if (lane == 59) {
+    // this point's order of the NEPS half-cells (degeneracy bin)
+    const double a = fabs(mu_e / T);
+    int bin = 0;
+    if (a >= kNpBinHi) bin = kNpBins - 1;
+    else if (a >= kNpBinLo) bin = np_bin_of(log_bounded(a) * 0.43429448190325182765);
+    s_in[kNpOffSlot] = static_cast<double>(bin * L.Q);
   }
 
   // ---- per-point constants --------------------------------------------------
@@ -499,6 +513,15 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
   // ====================== 2D: inelastic scattering on e-+ ===================
   if (A.flags.use_inelastic_scatt == 1) {
     const double bpl = c::NEPSBPlus, bze = c::NEPSBZero, bmi = c::NEPSBMinus;
+    const uint16_t* const order = reinterpret_cast<const uint16_t*>(tb + L.npp);
+    if constexpr (kPermLds) {
+      // 512 indices, 16 bytes per lane (L.npp is even and a bin's block is
+      // 1 KB: 16-byte aligned)
+      const int npo = __builtin_amdgcn_readfirstlane(static_cast<int>(vin[kNpOffSlot]));
+      s_perm[lane] = reinterpret_cast<const uint4*>(order + npo)[lane];
+      __syncthreads();
+    }
+    const uint16_t* const s_order = reinterpret_cast<const uint16_t*>(s_perm);
     for (int h = lane; h < L.Q; h += kWave) {
       // T and eta from LDS in every round, and the products formed from them
       // here: hoisted out of the loop they would hold registers through it
@@ -507,7 +530,15 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       const double pre = c::NEPSConst * (T * T);
       // t = 4 s_neps with s_neps = 2.182 T (m1_opacities.hpp:1092, 1167)
       const double ht = 0.5 * (4. * (c::PairScale * T));
-      const double* __restrict__ hr = tb + L.np + NP_REC * h;
+      // this point's order of visit (its degeneracy bin)
+      int hk;
+      if constexpr (kPermLds) {
+        hk = s_order[h];
+      } else {
+        const int npo = __builtin_amdgcn_readfirstlane(static_cast<int>(vin[kNpOffSlot]));
+        hk = order[npo + h];
+      }
+      const double* __restrict__ hr = tb + L.np + NP_REC * hk;
       // nu, nubar and w = nu / T with the reference's roundings
       // (m1_opacities.hpp:910-912, 982-984; kernel_nes.hpp:144-145) so that
       // the Taylor-branch tests of NESKernels / NPSKernels decide identically

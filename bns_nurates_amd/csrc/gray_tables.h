@@ -66,8 +66,28 @@ struct TableLayout {
   int n1, n2;   // 1D and 2D quadrature orders
   int N, P, Q;  // 2D nodes (2 n2), pair cells (N(N+1)/2), NEPS half-cells (2 n2^2)
   int q1, nd, cl, np;  // offsets (in doubles) of the four record arrays
+  int npp;             // offset (in doubles) of the half-cell orders: uint16 [kNpBins][Q]
   int total;           // doubles in the block
 };
+
+// The NEPS half-cells are visited in one order per range of the electron
+// degeneracy |eta| ("bin"; uint16 record indices at npp): a round of 64 lanes evaluates
+// a Fermi-integral bundle in its exponential form, its rational form or
+// both, as its lanes' arguments eta - v, -eta - v fall, and an order that
+// groups the half-cells by the forms they take at that |eta| has fewer
+// rounds that pay for both.  Bin 0: |eta| < 1e-3 (and NaN); bins 1..7:
+// 2.5 per decade up to 10^-0.2; bin 8: 10^-0.2 and above.
+constexpr int kNpBins = 9;
+constexpr double kNpBinLo = 1e-3, kNpBinHi = 0.63095734448019325, kNpBinsPerDecade = 2.5;
+
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline int np_bin_of(double abs_eta_log10) {
+  // abs_eta_log10 = log10(|eta|) for kNpBinLo <= |eta| < kNpBinHi
+  int b = 1 + static_cast<int>(kNpBinsPerDecade * (abs_eta_log10 + 3.0));
+  return b < 1 ? 1 : (b > kNpBins - 2 ? kNpBins - 2 : b);
+}
 
 // record counts and offsets for quadrature orders n1 (1D) and n2 (2D); a
 // constant expression, so that a kernel instance for fixed orders gets its
@@ -86,7 +106,8 @@ constexpr TableLayout gray_table_layout(int n1, int n2) {
   L.nd = L.q1 + Q1_REC * n1 + (n1 & 1) * Q1_REC;
   L.cl = L.nd + ND_REC * L.N;
   L.np = L.cl + CL_REC * L.P;
-  L.total = L.np + NP_REC * L.Q;
+  L.npp = (L.np + NP_REC * L.Q + 1) / 2 * 2;  // even: 16-byte aligned orders
+  L.total = L.npp + (kNpBins * L.Q + 3) / 4;
   return L;
 }
 
