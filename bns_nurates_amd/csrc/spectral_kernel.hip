@@ -298,7 +298,10 @@ k_spectral_f64(SpectralKernelArgs A) {
 
   const double a1e = c::PairA1E * c::PairA1E, a2e = c::PairA2E * c::PairA2E;
   const double a1x = c::PairA1X * c::PairA1X, a2x = c::PairA2X * c::PairA2X;
-  const int Gr = kWave / N;  // energies per round of cells
+  // energies per round of cells: a round takes one half of the partner nodes
+  // (x_a or 1 / x_a) for Gh energies, so that its lanes' partner energies,
+  // and with them the Fermi-integral forms and branches, stay closer
+  const int Gh = kWave / n;
 
   for (int64_t chunk = c_begin; chunk < c_end; ++chunk) {
     const int64_t e0 = chunk * E;
@@ -384,21 +387,19 @@ k_spectral_f64(SpectralKernelArgs A) {
     }
     __syncthreads();
 
-    // per-energy quadrature sums of one round (GaussLegendreIntegrate1D,
-    // integration.hpp:511-546): lane (g, k, h) sums integrand k of energy
-    // r0 + g over half h of the nodes in node order (DoIntegration,
+    // per-energy quadrature sums over one half h of the nodes
+    // (GaussLegendreIntegrate1D, integration.hpp:511-546): lane (g, k) sums
+    // integrand k of energy r0 + g over the half in node order (DoIntegration,
     // integration.hpp:95-108), loads in batches of four so that their LDS
-    // latencies overlap; the neighbouring lane holds the other half and lane
-    // h = 0 stores t (s1 + s2).  16 Gr <= 64 for n >= 4; the partner lanes
-    // j and j ^ 1 are in the same iteration for any n.
-    auto reduce_round = [&](int r0, int dst, bool neps) {
-      for (int j0 = 0; j0 < 16 * Gr; j0 += kWave) {
+    // latencies overlap; half 0 parks its sum s1 in the destination, half 1
+    // stores t (s1 + s2)
+    auto reduce_half = [&](int r0, int dst, bool neps, int h) {
+      for (int j0 = 0; j0 < 8 * Gh; j0 += kWave) {
         const int j = j0 + lane;
-        const int g = j >> 4, k = (j >> 1) & 7, h = j & 1, e = r0 + g;
-        const bool live = j < 16 * Gr && e < ne;
-        double s = 0.;
-        if (live) {
-          const double* f = V + L.sf + k * kWave + g * N + h * n;
+        const int g = j >> 3, k = j & 7, e = r0 + g;
+        if (j < 8 * Gh && e < ne) {
+          const double* f = V + L.sf + k * kWave + g * n;
+          double s = 0.;
           int i = 0;
           for (; i + 4 <= n; i += 4) {
             const double a0 = f[i], a1 = f[i + 1], a2 = f[i + 2], a3 = f[i + 3];
@@ -408,20 +409,22 @@ k_spectral_f64(SpectralKernelArgs A) {
             s = fma(s_w[i + 3], a3, s);
           }
           for (; i < n; ++i) s = fma(s_w[i], f[i], s);
-        }
-        const double other = __shfl_xor(s, 1, kWave);  // every lane takes part
-        if (live && h == 0) {
-          const double t = neps ? V[L.nu + e] : tpair;
-          V[dst + k * E + e] = t * (s + other);
+          double* d = V + dst + k * E + e;
+          if (h == 0) {
+            *d = s;
+          } else {
+            const double t = neps ? V[L.nu + e] : tpair;
+            *d = t * (*d + s);
+          }
         }
       }
     };
 
     // ====================== pair + bremsstrahlung integral ========================
     if (do_pb) {
-      for (int r0 = 0; r0 < ne; r0 += Gr) {
-        const int g = lane / N, a = lane - g * N, e = r0 + g;
-        if (g < Gr && e < ne) {
+      for (int r0 = 0; r0 < ne; r0 += Gh) for (int h = 0; h < 2; ++h) {
+        const int g = lane / n, a = lane - g * n + h * n, e = r0 + g;
+        if (g < Gh && e < ne) {
           const double nu = V[L.nu + e], y = V[L.y + e];
           const double nub = V[L.nub + a], z = V[L.z + a];
           double em[4] = {0., 0., 0., 0.}, ab[4] = {0., 0., 0., 0.};
@@ -510,7 +513,7 @@ k_spectral_f64(SpectralKernelArgs A) {
           }
         }
         __syncthreads();
-        reduce_round(r0, L.ip, false);
+        reduce_half(r0, L.ip, false, h);
         __syncthreads();
       }
     } else {
@@ -521,9 +524,9 @@ k_spectral_f64(SpectralKernelArgs A) {
     if (do_neps) {
       const double pre = c::NEPSConst * (T * T);
       const double bpl = c::NEPSBPlus, bze = c::NEPSBZero, bmi = c::NEPSBMinus;
-      for (int r0 = 0; r0 < ne; r0 += Gr) {
-        const int g = lane / N, a = lane - g * N, e = r0 + g;
-        if (g < Gr && e < ne) {
+      for (int r0 = 0; r0 < ne; r0 += Gh) for (int h = 0; h < 2; ++h) {
+        const int g = lane / n, a = lane - g * n + h * n, e = r0 + g;
+        if (g < Gh && e < ne) {
           const double nu = V[L.nu + e], w = V[L.y + e];
           const double xq = s_x[a < n ? a : a - n];
           // partner energy nu x or nu / x (t = nu, m1_opacities.hpp:1547)
@@ -623,7 +626,7 @@ k_spectral_f64(SpectralKernelArgs A) {
           }
         }
         __syncthreads();
-        reduce_round(r0, L.in, true);
+        reduce_half(r0, L.in, true, h);
         __syncthreads();
       }
     } else {
