@@ -314,7 +314,7 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
       hr_p = hp.p;
       hr_pinv = hp.pinv;
       hr_medium = hp.medium;
-      const HrChan hc = hr98_channel(lane < 3 ? lane : 0, T, vin[0], vin[2], vin[3], hp.y);
+      const HrChan hc = hr98_channel(lane < 3 ? lane : 0, T, vin[0], vin[2], vin[3], hp.y, hp.p);
       if (lane < 3) s_hc[lane] = hc;
     }
     if (lane == 0) {
@@ -457,17 +457,22 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
           double px12 = cr[CL_X12], px28 = cr[CL_X28];
           double xm = x - 1.2;
           double fx = 3. + xm * xm + cr[CL_XM4];
+          // the factors the three channels share: s_d^-p = e^{-p lsd_pt}
+          // e^{-p lsdx} with the first per point, 1 / (fx f_pt) and the
+          // second division of C with their cell reciprocals
+          // (operands normal, quotients O(1e-30..1e30))
+          const double sdx = exp_bounded(-hr_p * lsdx);
+          const double rfx = div_normal(1., fx);
+          const double r28 = div_normal(30., 30. + 0.005 * px28);
           double tot = 0.;
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
             lds_cv<HrChan>& h = lds_volatile(s_hc)[ch];
-            double sd = exp_bounded(-hr_p * (h.lsd_pt + lsdx));
+            double sd = h.sd_pt * sdx;
             double sbase = exp_bounded(hr_pinv * log_bounded(pnd + sd));
-            // the operands are normal and the quotients O(1e-30..1e30)
-            double fb = 1. + div_normal(1., fx * h.f_pt);
+            double fb = 1. + rfx * h.rf_pt;
             double gbm = 1. - px11 * h.g0;
-            double cb = div_normal(1.1 * px11 * h.h, 2.3 + h.h * px093 + 0.0001 * px12);
-            cb = div_normal(cb * 30., 30. + 0.005 * px28);
+            double cb = div_normal(1.1 * px11 * h.h, 2.3 + h.h * px093 + 0.0001 * px12) * r28;
             double sb = sbase * fb * (1. + cb * gbm);
             double gx = h.gamma_half_gb;
             double sabs = div_normal(h.gamma_over_T, x * x + gx * gx) * sb;

@@ -346,6 +346,8 @@ struct HrChan {
   double f_pt;    // (1 + es^2)(1 + y^4)
   double g0;      // c1 y / (0.8 + 0.06 y^1.05) sqrt(es) / (es + 0.2)
   double h;       // h_brem
+  double sd_pt;   // e^{-p lsd_pt}: s_d^-p = sd_pt e^{-p log(cell factor)}
+  double rf_pt;   // 1 / f_pt
 };
 
 // The per-point scalars of the HR98 fit shared by the three channels
@@ -377,7 +379,7 @@ __device__ __forceinline__ HrPoint hr98_point(double T, double nb, bool medium_c
 // x-independent parts of BremSingleChannelAbsKernel, BremKernelG and
 // BremKernelS (kernel_brem_HR98.hpp:33-338); y = HrPoint::y
 __device__ __forceinline__ HrChan hr98_channel(int ch, double T, double nb, double yp,
-                                               double yn, double y) {
+                                               double yn, double y, double p) {
   const double T10 = T * 0.1;
   double xmean = sqrt(yn * yp);
   double nNc = ch == 0 ? nb * yn : ch == 1 ? nb * yp : nb * xmean;
@@ -415,6 +417,8 @@ __device__ __forceinline__ HrChan hr98_channel(int ch, double T, double nb, doub
   HrChan h;
   h.lsd_pt = log(3. * c::PiHalfToFiveHalves * exp(-2.5 * log(eg)) * fu);
   h.f_pt = (1. + eg * eg) * (1. + y4);
+  h.sd_pt = exp_any(-p * h.lsd_pt);
+  h.rf_pt = 1. / h.f_pt;
   h.g0 = 0.0044 * ye / (0.8 + 0.06 * y105) * sqrt(eg) / (eg + 0.2);
   h.h = 0.1 * eg / (2.39 + 0.1 * exp(1.1 * log(eg)));
   h.gamma_half_gb = 0.5 * gamma * gb;

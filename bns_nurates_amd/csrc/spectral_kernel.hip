@@ -105,17 +105,20 @@ __device__ __forceinline__ double hr98_abs(double x, const HrPoint& hp,
   const double pnd = exp_bounded(-hp.p * log_bounded(snd));
   const double xm = xc - 1.2;
   const double fx = 3. + xm * xm + pxm4;
+  // the factors the three channels share (as in the gray kernel)
+  const double sdx = exp_bounded(-hp.p * lsdx);
+  const double rfx = div_normal(1., fx);
+  const double r28 = div_normal(30., 30. + 0.005 * px28);
   double tot = 0.;
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
-    const double lsd_pt = hc[ch].lsd_pt, f_pt = hc[ch].f_pt, g0 = hc[ch].g0, h = hc[ch].h;
+    const double sd_pt = hc[ch].sd_pt, rf_pt = hc[ch].rf_pt, g0 = hc[ch].g0, h = hc[ch].h;
     const double gx = hc[ch].gamma_half_gb, got = hc[ch].gamma_over_T, wt = hc[ch].weight;
-    double sd = exp_bounded(-hp.p * (lsd_pt + lsdx));
+    double sd = sd_pt * sdx;
     double sbase = exp_bounded(hp.pinv * log_bounded(pnd + sd));
-    double fb = 1. + div_normal(1., fx * f_pt);
+    double fb = 1. + rfx * rf_pt;
     double gbm = 1. - px11 * g0;
-    double cb = div_normal(div_normal(1.1 * px11 * h, 2.3 + h * px093 + 0.0001 * px12) * 30.,
-                           30. + 0.005 * px28);
+    double cb = div_normal(1.1 * px11 * h, 2.3 + h * px093 + 0.0001 * px12) * r28;
     double sb = sbase * fb * (1. + cb * gbm);
     tot += wt * (div_normal(got, x * x + gx * gx) * sb);
   }
@@ -211,7 +214,7 @@ k_spectral_f64(SpectralKernelArgs A) {
     brt_pre = c::HC6FourPiSq * c::BremC4BRT06 * (q / T7);
   } else if (do_brem && A.params.brem_impl == 0) {
     hp = hr98_point(T, vin[0], A.params.use_NN_medium_corr != 0);
-    const HrChan hc = hr98_channel(lane < 3 ? lane : 0, T, vin[0], vin[2], vin[3], hp.y);
+    const HrChan hc = hr98_channel(lane < 3 ? lane : 0, T, vin[0], vin[2], vin[3], hp.y, hp.p);
     if (lane < 3) s_hc[lane] = hc;
   }
   // the bremsstrahlung point constants stay in LDS and are read in the cell
