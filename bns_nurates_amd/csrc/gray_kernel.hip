@@ -598,9 +598,13 @@ __device__ __forceinline__ void gray_point(const GrayKernelArgs& A) {
             fdi_bundle<1, 5, true>(el - w, B5);
             fdi_bundle<3, 5, true>(el - ad, C3);
           } else {
-            fdi_bundle<1, 5>(el - wp, A5);
-            fdi_bundle<1, 5>(el - w, B5);
-            fdi_bundle<3, 5>(el - ad, C3);
+            // electron targets: from eta = 40 on whole rounds of arguments
+            // eta - v commonly lie beyond 40 on either side (the deep tail
+            // of the exponential form, reflected where positive)
+            const bool tail = el > 40.;
+            fdi_bundle<1, 5, true>(el - wp, A5, tail);
+            fdi_bundle<1, 5, true>(el - w, B5, tail);
+            fdi_bundle<3, 5, true>(el - ad, C3, tail);
           }
           double fw0 = A5[1] - B5[1], fw1 = A5[2] - B5[2], fw2 = A5[3] - B5[3];
           double fw3 = A5[4] - B5[4], fw4 = A5[5] - B5[5];

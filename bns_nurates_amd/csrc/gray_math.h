@@ -517,10 +517,10 @@ struct FdiReflect {
 // callers enable it where whole wavefronts commonly fall in the tail, since
 // the test itself costs a few instructions per bundle.
 template <int KLO, int KHI = 5, bool DEEP = false>
-BNS_HD void fdi_bundle(double y, double* F) {
+BNS_HD void fdi_bundle(double y, double* F, bool try_deep = true) {
   double x = -fabs(y);
 #if defined(__HIP_DEVICE_COMPILE__)
-  if (DEEP && __all(x < kFdiDeep)) {
+  if (DEEP && try_deep && __all(x < kFdiDeep)) {
     FdiDeepUnroll<KLO>::template eval<KHI>(exp_bounded(fmax(x, -746.)), F);
     if (y > 0.) FdiReflect<KLO>::template apply<KHI>(y, F);
     return;
