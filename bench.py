@@ -44,12 +44,12 @@ PEAK_FP64_VECTOR_TFLOPS = 78.6  # MI355X datasheet, FP64 vector (FMA = 2)
 # (profiles/k_gray_m1_f64_grid128_n16_brt06.md):
 # HBM bytes per point per launch, FETCH_SIZE (x2, calibrated on a 1 GiB copy)
 # + WRITE_SIZE (x1) over the 2,097,152-point launch ...
-TRAFFIC_BYTES_PER_POINT = (346762.0 + 327680) * 1024 / 2097152
+TRAFFIC_BYTES_PER_POINT = (346834.0 + 327680) * 1024 / 2097152
 # ... against the algorithmic 21 input + 20 output doubles per point
 ALGORITHMIC_BYTES_PER_POINT = 8 * (21 + 20)
 # hardware FP64 FLOPs per point, (ADD + MUL + TRANS + 2 FMA) x 64 lanes from
 # the SQ_INSTS_VALU_*_F64 counters over the 2,097,152-point launch
-HW_FLOPS_PER_POINT = 1408132.7
+HW_FLOPS_PER_POINT = 1399451.8
 
 
 # The spectral path (ComputeSpectralOpacitiesNotStimulatedAbs): 32^3 points
