@@ -145,7 +145,8 @@ __global__ __launch_bounds__(256) void k_gemm_v(int M, int N, int K,
                                                 const uint16_t* __restrict__ A, int64_t lda,
                                                 const uint16_t* __restrict__ B, int64_t ldb,
                                                 float* __restrict__ C, int64_t ldc, int kchunk,
-                                                int64_t csplit) {
+                                                int64_t csplit, const float* __restrict__ bias,
+                                                int act, uint16_t* __restrict__ H) {
   __shared__ __attribute__((aligned(16))) uint16_t As[64][40];  // [m][k]
   __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];  // [n][k]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -214,8 +215,17 @@ __global__ __launch_bounds__(256) void k_gemm_v(int M, int N, int K,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v, col = n0 + wn + j * 16 + (lane & 15);
-        if (row < M && col < N)
-          (SPLIT ? C + blockIdx.z * csplit : C)[int64_t(row) * ldc + col] = acc[i][j][v];
+        if (row < M && col < N) {
+          if (!SPLIT && bias) {
+            // the forward pass's bias and activation (what k_bias_act does
+            // to the stored product, the same operations)
+            const float z = acc[i][j][v] + bias[col];
+            C[int64_t(row) * ldc + col] = z;
+            if (H) H[int64_t(row) * ldc + col] = f2bf(act ? silu(z) : z);
+          } else {
+            (SPLIT ? C + blockIdx.z * csplit : C)[int64_t(row) * ldc + col] = acc[i][j][v];
+          }
+        }
       }
 }
 
@@ -813,38 +823,44 @@ int mlp_fail(bnsamd_mlp* m, int code, const std::string& msg) {
 template <bool SPLIT>
 void gemm_v(hipStream_t s, dim3 grid, bool ak, bool bk, int M, int N, int K, const uint16_t* A,
             int64_t lda, const uint16_t* B, int64_t ldb, float* C, int64_t ldc, int kchunk,
-            int64_t csplit) {
+            int64_t csplit, const float* bias, int act, uint16_t* H) {
   const dim3 block(256);
   if (ak && bk)
     hipLaunchKernelGGL((k_gemm_v<SPLIT, true, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb,
-                       C, ldc, kchunk, csplit);
+                       C, ldc, kchunk, csplit, bias, act, H);
   else if (ak)
     hipLaunchKernelGGL((k_gemm_v<SPLIT, true, false>), grid, block, 0, s, M, N, K, A, lda, B,
-                       ldb, C, ldc, kchunk, csplit);
+                       ldb, C, ldc, kchunk, csplit, bias, act, H);
   else if (bk)
     hipLaunchKernelGGL((k_gemm_v<SPLIT, false, true>), grid, block, 0, s, M, N, K, A, lda, B,
-                       ldb, C, ldc, kchunk, csplit);
+                       ldb, C, ldc, kchunk, csplit, bias, act, H);
   else
     hipLaunchKernelGGL((k_gemm_v<SPLIT, false, false>), grid, block, 0, s, M, N, K, A, lda, B,
-                       ldb, C, ldc, kchunk, csplit);
+                       ldb, C, ldc, kchunk, csplit, bias, act, H);
 }
 
 // C = A B, or with splitk > 1 C += A B as splitk partial products over
 // chunks of K written to part (room for kMaxSplits x M x N floats; C then
-// M x N contiguous, ldc = N) and summed in chunk order
+// M x N contiguous, ldc = N) and summed in chunk order.  With bias: C = A B
+// + bias per column and H = BF16(act ? SiLU(C) : C) (ldc = N), in the
+// product's epilogue where it has one (else by k_bias_act)
 void gemm(hipStream_t s, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak,
           const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, int splitk,
-          float* part = nullptr) {
+          float* part = nullptr, const float* bias = nullptr, int act = 0,
+          uint16_t* H = nullptr) {
   int kchunk = (K + splitk - 1) / splitk;
   kchunk = (kchunk + 31) / 32 * 32;
   int z = (K + kchunk - 1) / kchunk;
   dim3 grid((N + 63) / 64, (M + 63) / 64, z), block(256);
   const int64_t mn = int64_t(M) * N;
   float* const Cz = z > 1 ? part : C;  // where the kernel writes
-  auto finish = [&]() {
+  auto finish = [&](bool epilogue_done) {
     if (z > 1)
       hipLaunchKernelGGL(k_sum_splits, dim3(unsigned((mn + 255) / 256)), dim3(256), 0, s, mn, z,
                          part, C);
+    if (bias && !epilogue_done)
+      hipLaunchKernelGGL(k_bias_act, dim3((M + kEwRows - 1) / kEwRows), dim3(256), 0, s, M, N, C,
+                         bias, act, H);
   };
   // 16-byte pieces: the contiguous dimension of each operand has stride 1,
   // its extent and the other stride are multiples of 8, the bases are
@@ -856,9 +872,14 @@ void gemm(hipStream_t s, int M, int N, int K, const uint16_t* A, int64_t sam, in
                    ldb % 8 == 0 && (ak ? K % 8 == 0 : M % 8 == 0) &&
                    (bk ? K % 8 == 0 : N % 8 == 0);
   if (vec) {
-    if (z > 1) gemm_v<true>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, Cz, ldc, kchunk, mn);
-    else gemm_v<false>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, Cz, ldc, kchunk, 0);
-    finish();
+    if (z > 1) {
+      gemm_v<true>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, Cz, ldc, kchunk, mn, nullptr, 0,
+                   nullptr);
+      finish(false);
+    } else {
+      gemm_v<false>(s, grid, ak, bk, M, N, K, A, lda, B, ldb, Cz, ldc, kchunk, 0, bias, act, H);
+      finish(true);
+    }
     return;
   }
   if (z > 1)
@@ -867,7 +888,7 @@ void gemm(hipStream_t s, int M, int N, int K, const uint16_t* A, int64_t sam, in
   else
     hipLaunchKernelGGL(k_gemm<false>, grid, block, 0, s, M, N, K, A, sam, sak, B, sbk, sbn, Cz,
                        ldc, kchunk, int64_t(0));
-  finish();
+  finish(false);
 }
 
 void fill(hipStream_t s, int64_t n, float* p, float v) {
@@ -881,10 +902,9 @@ void forward(bnsamd_mlp* m, Work& W, int e, int B) {
   const uint16_t* in = W.Xb;
   for (int l = 0; l < kLayers; ++l) {
     const int K = kLayerK[l], N = kLayerN[l];
-    gemm(s, B, N, K, in, K, 1, m->wb[e] + kWOff[l], N, 1, W.Z[l], N, 1);
-    hipLaunchKernelGGL(k_bias_act, dim3((B + kEwRows - 1) / kEwRows), dim3(256), 0, s, B, N, W.Z[l],
-                       m->w[e] + kBOff[l], l + 1 < kLayers ? 1 : 0,
-                       l + 1 < kLayers ? W.H[l] : static_cast<uint16_t*>(nullptr));
+    gemm(s, B, N, K, in, K, 1, m->wb[e] + kWOff[l], N, 1, W.Z[l], N, 1, nullptr,
+         m->w[e] + kBOff[l], l + 1 < kLayers ? 1 : 0,
+         l + 1 < kLayers ? W.H[l] : static_cast<uint16_t*>(nullptr));
     in = W.H[l];
   }
 }
