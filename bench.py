@@ -259,7 +259,7 @@ def serving_sweep(templates=96, fractions=(0.3, 0.6, 0.9), seconds=2.0, seed=3):
 
 
 def surrogate_eval(ctx, dpts, ref, fp64_ms, params, nquad, train_points, epochs, tols,
-                   ln_loss=False, steps=3, coverage=0.999, judge_epochs=16):
+                   ln_loss=False, steps=3, coverage=0.999, judge_epochs=16, cal_frac=0.5):
     """The learned form: an ensemble of 4 MLPs trained on the GPU (own
     kernels) on train_points seeded points (seed 1000, disjoint from the
     evaluation grid) with FP64-path targets at order nquad; for each relative
@@ -321,7 +321,8 @@ def surrogate_eval(ctx, dpts, ref, fp64_ms, params, nquad, train_points, epochs,
         spread_rule = S.calibrate(ens, ctx, tol, params, nquad, coverage=coverage, log=log,
                                   data=half)
         cal = S.calibrate_judge(ens, ctx, tol, params, nquad, data=caldata, pts=calpts,
-                                coverage=coverage, epochs=judge_epochs, log=log)
+                                coverage=coverage, epochs=judge_epochs, log=log,
+                                cal_frac=cal_frac)
         cal["spread_rule_split_B"] = spread_rule["split_B"]
         bound = float(np.log1p(tol))
         ctx.set_quadrature(nquad)
@@ -690,6 +691,9 @@ def main():
     ap.add_argument("--surrogate-epochs", type=int, default=96)
     ap.add_argument("--surrogate-judge-epochs", type=int, default=16,
                     help="training epochs of each tolerance's acceptance judge")
+    ap.add_argument("--surrogate-cal-frac", type=float, default=0.75,
+                    help="violators the judge's threshold allows on its calibration split, "
+                         "as a fraction of 1 - coverage")
     ap.add_argument("--surrogate-std-loss", action="store_true",
                     help="train the learned form on the squared error in per-output "
                          "standardised units instead of natural-log units")
@@ -858,7 +862,8 @@ def main():
                                            args.surrogate_train_points, args.surrogate_epochs,
                                            tols=(1e-2, 3e-2, 1e-1),
                                            ln_loss=not args.surrogate_std_loss,
-                                           judge_epochs=args.surrogate_judge_epochs)
+                                           judge_epochs=args.surrogate_judge_epochs,
+                                           cal_frac=args.surrogate_cal_frac)
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(pts, args.nquad, W.params_default_all(W.BREM_BRT06))

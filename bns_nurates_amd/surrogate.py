@@ -653,11 +653,12 @@ def judge_scores(j: "Ensemble", ctx, pts: np.ndarray, outside: np.ndarray) -> np
 
 def calibrate_judge(ens: "Ensemble", ctx, tol: float, params, nquad: int, data=None,
                     pts=None, ncal: int = 1 << 22, seed: int = 2000, coverage: float = 0.999,
-                    minacc: int = 3000, epochs: int = 16, log=None) -> dict:
+                    minacc: int = 3000, epochs: int = 16, log=None,
+                    cal_frac: float = 0.5) -> dict:
     """Trains a judge for tol on the first half of ncal held-out points,
     calibrates its threshold on the third quarter (the largest that accepts
-    at most (1 - coverage) / 2 points with an output beyond the bar, and at
-    least minacc points), reports on the last quarter, and sets it on the
+    at most cal_frac (1 - coverage) points with an output beyond the bar, and
+    at least minacc points), reports on the last quarter, and sets it on the
     ensemble (the spread rule off: err_a = err_k = 0).  data / pts:
     calibration_data() and the points it was made on (seed `seed`)."""
     if data is None:
@@ -670,7 +671,7 @@ def calibrate_judge(ens: "Ensemble", ctx, tol: float, params, nquad: int, data=N
     ctx.set_quadrature(nquad)
     _, bad, _, _ = prediction_errors(mean, spread, outside, ref, noise, tol)
     anybad = np.any(bad, axis=0)
-    target = (1.0 - coverage) / 2
+    target = cal_frac * (1.0 - coverage)
     sA2 = judge_scores(j, ctx, pts[:, A2], outside[A2])
     t = _accept_threshold(sA2, anybad[A2], target, minacc)
     rep = {"tol": tol, "rule": "judge", "coverage_target": coverage, "calibration_points": n,
